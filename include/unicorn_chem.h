@@ -1,0 +1,137 @@
+/* unicorn_chem.h -- C ABI of the MI355X (gfx950) chemistry step for UNICORN.
+ *
+ * Two layers:
+ *
+ * 1. The Fortran drop-in (ISO_C_BINDING / plain symbol override).  These keep
+ *    the reference routines' names, argument lists and COMMON-block array
+ *    layout; a UNICORN build links them instead of INCLUDE-ing
+ *    src/lib/ckrates.f, solvesp.f and solveh.f (src/lib/spgsolv.f:2-4):
+ *
+ *      ckrates_  replaces SUBROUTINE CKRATES                      (src/lib/ckrates.f:1)
+ *      solvesp_  replaces SUBROUTINE SOLVESP(ISOR1,RELXSP,TOLRSP,SPNORM)
+ *                                                                 (src/lib/solvesp.f:1)
+ *      solveh_   replaces SUBROUTINE SOLVEH(ISOR2,RELXH,TOLRH,SIGSP,HNORM)
+ *                                                                 (src/lib/solveh.f:1)
+ *
+ *    They live in libunicorn_fshim.so (unicorn_amd/csrc/fortran_shim.cpp),
+ *    built for the grid PARAMETER(LI,LJ) of the Fortran build, and read and
+ *    write the COMMON blocks /CB01/ /CB02/ /CB03/ /CB04/ /CB06/ /CB08/ /REAC/
+ *    /HFORM/ /HRLS/ /DUMD/ /SOOT/ /DUMY/ /DUMS/ /DUMR/ /DUMMY/ /DUM0/ /DUMX/
+ *    exactly as the reference routines do.  Only the chemistry-only contract
+ *    (all transport coefficients E/W/P/Q, ZE..ZS, STE..STS zero) is executed
+ *    on the GPU; a call with non-zero coefficients stops the program with a
+ *    message (there is no CPU fallback).
+ *
+ * 2. The device API (libunicorn_chem.so), used by the shim and directly by
+ *    host programs: a context holds the scales and scratch, a step advances a
+ *    device-resident field by one chemistry step.
+ *
+ * Field layout (both layers): the reference's, i.e. for an LI x LJ grid the
+ * flat index is c = (I-1)*LJ + (J-1) (J fastest); species arrays are
+ * [52][LI*LJ].  Temperature TK and enthalpy HT are the reference's
+ * nondimensional variables (T/TSTR, h/VSTR^2).
+ *
+ * Errors: every function returns 0 on success or a negative UC_E_* code; the
+ * message of the last error on the calling thread is uc_last_error().
+ */
+#ifndef UNICORN_CHEM_H
+#define UNICORN_CHEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UC_ABI_VERSION 1
+
+enum {
+    UC_OK = 0,
+    UC_E_ARG = -1,       /* bad argument (sizes, null pointers, unknown mode) */
+    UC_E_HIP = -2,       /* a HIP runtime call failed */
+    UC_E_NOGPU = -3,     /* no gfx950 device */
+    UC_E_UNSUPPORTED = -4 /* input outside the implemented contract */
+};
+
+/* sweep control: UC_MODE_GLOBAL reproduces the reference's global stops
+ * (solvesp.f:2834, solveh.f:1162); UC_MODE_PERCELL stops each cell on its
+ * own mean |dY| <= TOLRSP and |dh| <= TOLRH. */
+enum { UC_MODE_GLOBAL = 0, UC_MODE_PERCELL = 2 };
+
+/* /CB06/ scales, /CB02/ WM and /HFORM/ HFO as the reference's main program
+ * sets them (src/unicornd-mild-sd-5.f:421-464). */
+typedef struct {
+    double alstr, vstr, rstr, pstr, tstr, tmstr, amstr, acstr, dstr, hstr,
+        cpstr, akstr, epstr, wmstr, gasc, beta1, beta2, beta3, beta4, rei;
+    double pback;     /* PREF/PSTR (main:429) */
+    double wm[52];    /* W_k/1000/WMSTR */
+    double hfo[52];   /* HCONV*a6_low(k)/WM(k) */
+} uc_scales;
+
+/* Solver controls of one step (the SPGSOLV arguments, spgsolv.f:6-7). */
+typedef struct {
+    double dt;        /* /CB03/ DT, nondimensional */
+    double relxsp, tolrsp, relxh, tolrh;
+    int32_t isor1, isor2; /* maximum species / enthalpy sweeps */
+    int32_t mode;         /* UC_MODE_* */
+    int32_t predict;      /* global mode: predicted sweep count (0 = isor1) */
+} uc_controls;
+
+/* Per-step results. */
+typedef struct {
+    int32_t isor1, isor2;   /* sweeps used, reference convention (max+1 = not converged) */
+    double resdc, resdh;    /* SPGSOLV residuals (spgsolv.f:1299-1416) */
+    int64_t cell_sweeps;    /* species sweeps executed, summed over cells */
+    int64_t redo_cells;     /* cells recomputed after the global stop was known */
+    double ms_rates_sweeps, ms_total; /* device time of the fused kernel, of the step */
+} uc_stats;
+
+/* Device-resident field (all pointers are device pointers). */
+typedef struct {
+    int32_t li, lj;         /* grid, including the boundary frame */
+    double *rho;            /* [n] */
+    double *y;              /* [52][n] */
+    double *tk, *ht;        /* [n] */
+    double *stmf, *stnd;    /* [n] soot mass fraction, number density */
+    const int32_t *iskip;   /* [n] >= 3: skipped by the chemistry (ckrates.f:93) */
+    double *qdot;           /* [n] out: heat release (solveh.f:1142) */
+    double *qdsp;           /* [52][n] out: species production (solveh.f:1137) */
+} uc_field;
+
+typedef struct uc_context uc_context;
+
+int uc_abi_version(void);
+const char *uc_last_error(void);
+
+/* Scales from the reference-values card (main:156): VREF, TREF, PREF, the
+ * geometry card's RTIN/RTOT, FO2IN and IGRAV. */
+int uc_scales_from_card(double vref, double tref, double pref, double rtin, double rtot,
+                        double fo2in, int32_t igrav, uc_scales *out);
+
+/* A context for fields of up to max_cells cells on the current HIP device.
+ * It owns the scratch that the fused kernel needs (per-thread RC slabs,
+ * sweep-residual partial sums); it allocates with hipMalloc. */
+int uc_create(const uc_scales *scales, int64_t max_cells, uc_context **out);
+int uc_destroy(uc_context *ctx);
+
+/* One chemistry step on a device-resident field, on `stream` (a hipStream_t,
+ * NULL = default stream).  Sequence (the reference around SPGSOLV):
+ * main:821-828 clip/closure, CKRATES, SOLVESP, SOLVEH, the SPGSOLV epilogue,
+ * and the ideal-gas density update main:1056-1073.  Synchronises the stream
+ * (the global stops are decided on the host). */
+int uc_step(uc_context *ctx, const uc_field *field, const uc_controls *ctl, void *stream,
+            uc_stats *stats);
+
+/* Same, on host arrays (copies in and out; the PCIe cost is in ms_total). */
+int uc_step_host(uc_context *ctx, const uc_field *host_field, const uc_controls *ctl,
+                 uc_stats *stats);
+
+/* ---- Fortran drop-in (libunicorn_fshim.so) ------------------------------ */
+void ckrates_(void);
+void solvesp_(int32_t *isor1, double *relxsp, double *tolrsp, double *spnorm);
+void solveh_(int32_t *isor2, double *relxh, double *tolrh, double *sigsp, double *hnorm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

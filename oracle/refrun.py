@@ -1,0 +1,108 @@
+"""Test infrastructure (oracle): NOT part of the shipped product.
+
+Runs the flang-compiled reference chemistry (oracle/_ref/ref_driver, built by
+oracle/Makefile from src/lib/{ckrates,solvesp,solveh,DATA}.f) on a field and
+returns its outputs.  File layout: see oracle/ref_driver.c.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+DRIVER = HERE / "_ref" / "ref_driver"
+LSP, LRX = 52, 544
+FLAG_FORCE_SWEEPS = 1
+
+
+def available() -> bool:
+    return DRIVER.exists()
+
+
+@dataclass
+class RefResult:
+    isor1: np.ndarray          # (nsteps,) species sweeps used per step
+    isor2: np.ndarray          # (nsteps,) enthalpy sweeps used per step
+    resdc: np.ndarray
+    resdh: np.ndarray
+    t_ck: np.ndarray           # seconds in CKRATES per step
+    t_sp: np.ndarray           # seconds in SOLVESP per step
+    t_sh: np.ndarray           # seconds in SOLVEH per step
+    rc: np.ndarray | None      # (ndump, 544) RC of the dumped cells after step 1's CKRATES
+    sour: np.ndarray | None    # (ndump, 6)
+    rho: np.ndarray
+    y: np.ndarray              # (52, n)
+    tk: np.ndarray
+    ht: np.ndarray
+    stmf: np.ndarray
+    stnd: np.ndarray
+    qdot: np.ndarray
+    qdsp: np.ndarray           # (52, n)
+    t_chem: float
+
+
+def write_input(path, field, nsteps=1, isor1=100, isor2=100, relxsp=0.99, tolrsp=1e-8,
+                relxh=0.99, tolrh=1e-8, force=False, dump_cells=(), ref_values=None):
+    from unicorn_amd.scales import Scales
+    sc = Scales()
+    rv = ref_values or [sc.vref, sc.tref, sc.pref, sc.rtin, sc.rtot, sc.fo2in]
+    dump_cells = np.asarray(dump_cells, dtype=np.int64)
+    with open(path, "wb") as f:
+        hdr = np.array([0x31534355, field.lj, field.li, nsteps, isor1, isor2,
+                        FLAG_FORCE_SWEEPS if force else 0, len(dump_cells)], dtype="<i8")
+        f.write(hdr.tobytes())
+        f.write(np.asarray(rv, dtype="<f8").tobytes())
+        f.write(np.array([field.dt, relxsp, tolrsp, relxh, tolrh], dtype="<f8").tobytes())
+        z = np.zeros(field.n)
+        for a in (field.rho, field.y, field.tk, field.ht, z, z, z, field.stmf, field.stnd):
+            f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+        f.write(np.ascontiguousarray(field.iskip, dtype="<i4").tobytes())
+        f.write(dump_cells.tobytes())
+
+
+def read_output(path, field, nsteps, ndump) -> RefResult:
+    raw = Path(path).read_bytes()
+    off = 8
+    assert int(np.frombuffer(raw, "<i8", 1, 0)[0]) == 0x314F4355
+    rc = sour = None
+    per = []
+    for s in range(nsteps):
+        if s == 0 and ndump:
+            blk = np.frombuffer(raw, "<f8", ndump * (LRX + 6), off).reshape(ndump, LRX + 6)
+            off += blk.nbytes
+            rc, sour = blk[:, :LRX].copy(), blk[:, LRX:].copy()
+        cnt = np.frombuffer(raw, "<i8", 2, off); off += 16
+        res = np.frombuffer(raw, "<f8", 5, off); off += 40
+        per.append((cnt, res))
+    n = field.n
+
+    def take(k=1):
+        nonlocal off
+        a = np.frombuffer(raw, "<f8", n * k, off).copy()
+        off += a.nbytes
+        return a.reshape(k, n) if k > 1 else a
+
+    rho = take(); y = take(LSP); tk = take(); ht = take(); stmf = take(); stnd = take()
+    qdot = take(); qdsp = take(LSP)
+    t_chem = float(np.frombuffer(raw, "<f8", 1, off)[0]); off += 8
+    assert off == len(raw), (off, len(raw))
+    return RefResult(np.array([c[0] for c, _ in per]), np.array([c[1] for c, _ in per]),
+                     np.array([r[0] for _, r in per]), np.array([r[1] for _, r in per]),
+                     np.array([r[2] for _, r in per]), np.array([r[3] for _, r in per]),
+                     np.array([r[4] for _, r in per]), rc, sour, rho, y, tk, ht, stmf, stnd,
+                     qdot, qdsp, t_chem)
+
+
+def run(field, nsteps=1, dump_cells=(), **kw) -> RefResult:
+    if not available():
+        raise FileNotFoundError(f"{DRIVER} not built (make -C oracle ref)")
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        write_input(fin, field, nsteps=nsteps, dump_cells=dump_cells, **kw)
+        subprocess.run([str(DRIVER), "run", fin, fout], check=True)
+        return read_output(fout, field, nsteps, len(dump_cells))

@@ -1,0 +1,205 @@
+"""Python host layer over the C ABI of the gfx950 chemistry step.
+
+``libunicorn_chem.so`` (built by ``unicorn_amd.build``) exports the functions
+declared in ``include/unicorn_chem.h``; this module binds them with ctypes and
+uses torch only for device memory and streams.  There is no CPU fallback: if
+the library or a GPU is missing, the calls raise.
+
+Host-side mirror of the reference's chemistry entry point
+(``SPGSOLV(ISOR1,RELXSP,TOLRSP,RESDC,ISOR2,RELXH,TOLRH,RESDH,SIGH,SIGSP)``,
+src/lib/spgsolv.f:6-7): ``ChemStep.step`` takes the same controls (sweep limits,
+relaxation factors, tolerances) and returns the same outputs (sweeps used,
+RESDC, RESDH), advancing the state in place.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from .fields import ChemField
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libunicorn_chem.so"
+MODE_GLOBAL, MODE_PERCELL = 0, 2
+
+
+class UcScales(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "alstr", "vstr", "rstr", "pstr", "tstr", "tmstr", "amstr", "acstr", "dstr", "hstr",
+        "cpstr", "akstr", "epstr", "wmstr", "gasc", "beta1", "beta2", "beta3", "beta4", "rei",
+        "pback")] + [("wm", C.c_double * 52), ("hfo", C.c_double * 52)]
+
+
+class UcControls(C.Structure):
+    _fields_ = [("dt", C.c_double), ("relxsp", C.c_double), ("tolrsp", C.c_double),
+                ("relxh", C.c_double), ("tolrh", C.c_double), ("isor1", C.c_int32),
+                ("isor2", C.c_int32), ("mode", C.c_int32), ("predict", C.c_int32)]
+
+
+class UcStats(C.Structure):
+    _fields_ = [("isor1", C.c_int32), ("isor2", C.c_int32), ("resdc", C.c_double),
+                ("resdh", C.c_double), ("cell_sweeps", C.c_int64), ("redo_cells", C.c_int64),
+                ("ms_rates_sweeps", C.c_double), ("ms_total", C.c_double)]
+
+
+class UcField(C.Structure):
+    _fields_ = [("li", C.c_int32), ("lj", C.c_int32)] + [
+        (n, C.c_void_p) for n in ("rho", "y", "tk", "ht", "stmf", "stnd", "iskip", "qdot", "qdsp")]
+
+
+EXPORTS = ("uc_abi_version", "uc_last_error", "uc_scales_from_card", "uc_create", "uc_destroy",
+           "uc_step", "uc_step_host")
+
+_lib = None
+
+
+class ChemError(RuntimeError):
+    pass
+
+
+def library() -> C.CDLL:
+    """Load the in-tree library (after torch, so both share one HIP runtime)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    try:
+        import torch  # noqa: F401  (binds libamdhip64.so.7 first)
+    except ImportError:
+        pass
+    if not LIB_PATH.exists():
+        raise ChemError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(str(LIB_PATH))
+    lib.uc_last_error.restype = C.c_char_p
+    for name in ("uc_scales_from_card", "uc_create", "uc_destroy", "uc_step", "uc_step_host"):
+        getattr(lib, name).restype = C.c_int
+    lib.uc_scales_from_card.argtypes = [C.c_double] * 6 + [C.c_int32, C.POINTER(UcScales)]
+    lib.uc_create.argtypes = [C.POINTER(UcScales), C.c_int64, C.POINTER(C.c_void_p)]
+    lib.uc_destroy.argtypes = [C.c_void_p]
+    lib.uc_step.argtypes = [C.c_void_p, C.POINTER(UcField), C.POINTER(UcControls), C.c_void_p,
+                            C.POINTER(UcStats)]
+    lib.uc_step_host.argtypes = [C.c_void_p, C.POINTER(UcField), C.POINTER(UcControls),
+                                 C.POINTER(UcStats)]
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise ChemError(f"unicorn_chem error {rc}: {library().uc_last_error().decode()}")
+
+
+def scales_from_card(vref=10.0, tref=294.0, pref=1.0133e5, rtin=0.0, rtot=0.100, fo2in=0.233,
+                     igrav=1) -> UcScales:
+    s = UcScales()
+    _check(library().uc_scales_from_card(vref, tref, pref, rtin, rtot, fo2in, igrav, C.byref(s)))
+    return s
+
+
+@dataclass
+class Controls:
+    """SPGSOLV's solver controls (docs/input.uni:25-27 defaults)."""
+    dt: float
+    isor1: int = 100
+    relxsp: float = 0.99
+    tolrsp: float = 1e-8
+    isor2: int = 100
+    relxh: float = 0.99
+    tolrh: float = 1e-8
+    mode: int = MODE_GLOBAL
+
+    def c(self) -> UcControls:
+        return UcControls(self.dt, self.relxsp, self.tolrsp, self.relxh, self.tolrh,
+                          self.isor1, self.isor2, self.mode, 0)
+
+
+@dataclass
+class StepStats:
+    isor1: int
+    isor2: int
+    resdc: float
+    resdh: float
+    cell_sweeps: int
+    redo_cells: int
+    ms_rates_sweeps: float
+    ms_total: float
+
+    @classmethod
+    def of(cls, s: UcStats) -> "StepStats":
+        return cls(s.isor1, s.isor2, s.resdc, s.resdh, s.cell_sweeps, s.redo_cells,
+                   s.ms_rates_sweeps, s.ms_total)
+
+
+class DeviceField:
+    """A ChemField resident in device memory (torch tensors on the current GPU)."""
+
+    def __init__(self, f: ChemField, device="cuda"):
+        import torch
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+        self.li, self.lj, self.dt, self.fuel = f.li, f.lj, f.dt, f.fuel
+        self.rho, self.y, self.tk, self.ht = t(f.rho), t(f.y), t(f.tk), t(f.ht)
+        self.stmf, self.stnd = t(f.stmf), t(f.stnd)
+        self.iskip = t(f.iskip.astype(np.int32))
+        self.qdot = torch.zeros(f.n, dtype=torch.float64, device=device)
+        self.qdsp = torch.zeros((52, f.n), dtype=torch.float64, device=device)
+
+    def c(self) -> UcField:
+        return UcField(self.li, self.lj, self.rho.data_ptr(), self.y.data_ptr(), self.tk.data_ptr(),
+                       self.ht.data_ptr(), self.stmf.data_ptr(), self.stnd.data_ptr(),
+                       self.iskip.data_ptr(), self.qdot.data_ptr(), self.qdsp.data_ptr())
+
+    def to_host(self) -> ChemField:
+        g = lambda a: a.detach().cpu().numpy().copy()  # noqa: E731
+        return ChemField(self.li, self.lj, g(self.rho), g(self.y), g(self.tk), g(self.ht), g(self.stmf),
+                         g(self.stnd), g(self.iskip), self.dt, self.fuel)
+
+
+class ChemStep:
+    """A chemistry-step context on the current GPU (scales + scratch)."""
+
+    def __init__(self, max_cells: int, scales: UcScales | None = None):
+        self.lib = library()
+        self.scales = scales or scales_from_card()
+        h = C.c_void_p()
+        _check(self.lib.uc_create(C.byref(self.scales), int(max_cells), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.uc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, f: DeviceField, ctl: Controls, stream=None) -> StepStats:
+        """Advance a device-resident field by one chemistry step."""
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        st = UcStats()
+        fc = f.c()
+        cc = ctl.c()
+        _check(self.lib.uc_step(self.h, C.byref(fc), C.byref(cc), C.c_void_p(stream), C.byref(st)))
+        return StepStats.of(st)
+
+    def step_host(self, f: ChemField, ctl: Controls) -> tuple[StepStats, np.ndarray, np.ndarray]:
+        """Advance a host field (copied to and from the GPU); returns stats, QDOT, QDSP."""
+        for name in ("rho", "y", "tk", "ht", "stmf", "stnd"):
+            a = getattr(f, name)
+            if a.dtype != np.float64 or not a.flags.c_contiguous:
+                setattr(f, name, np.ascontiguousarray(a, dtype=np.float64))
+        isk = np.ascontiguousarray(f.iskip, dtype=np.int32)
+        qdot = np.zeros(f.n)
+        qdsp = np.zeros((52, f.n))
+        p = lambda a: a.ctypes.data  # noqa: E731
+        fc = UcField(f.li, f.lj, p(f.rho), p(f.y), p(f.tk), p(f.ht), p(f.stmf), p(f.stnd), p(isk),
+                     p(qdot), p(qdsp))
+        st = UcStats()
+        cc = ctl.c()
+        _check(self.lib.uc_step_host(self.h, C.byref(fc), C.byref(cc), C.byref(st)))
+        return StepStats.of(st), qdot, qdsp
