@@ -480,6 +480,7 @@ struct uc_context {
   int64_t rsd_part_len = 0;
   double* res_part = nullptr;
   int64_t res_part_len = 0;
+  double* rows = nullptr;  // per-sweep residual sums (<= 1024 sweeps)
   hipEvent_t ev[4]{};
   double last_dt = -1.0;
   // cell list cache
@@ -544,6 +545,7 @@ int uc_create(const uc_scales* scales, int64_t max_cells, uc_context** out) {
   HIPCHK(hipMalloc(&ctx->rsd_part, sizeof(double) * ctx->rsd_part_len));
   ctx->res_part_len = 2 * ((max_cells + 255) / 256 + 1);
   HIPCHK(hipMalloc(&ctx->res_part, sizeof(double) * ctx->res_part_len));
+  HIPCHK(hipMalloc(&ctx->rows, sizeof(double) * 1024));
   for (auto& e : ctx->ev) HIPCHK(hipEventCreate(&e));
   *out = ctx.release();
   return UC_OK;
@@ -553,7 +555,7 @@ int uc_destroy(uc_context* ctx) {
   if (!ctx) return UC_OK;
   for (void* p : {(void*)ctx->dp, (void*)ctx->rcs, (void*)ctx->sa, (void*)ctx->sst, (void*)ctx->rhsz,
                   (void*)ctx->nsw, (void*)ctx->cells, (void*)ctx->redo, (void*)ctx->done,
-                  (void*)ctx->rsd_part, (void*)ctx->res_part})
+                  (void*)ctx->rsd_part, (void*)ctx->res_part, (void*)ctx->rows})
     (void)hipFree(p);
   for (auto& e : ctx->ev) (void)hipEventDestroy(e);
   delete ctx;
