@@ -1,0 +1,167 @@
+"""Benchmark: UNICORN's chemistry step on one MI355X.
+
+Workload (one "step" = one chemistry step of the whole field, inputs resident
+in HBM): the 1001 x 401 axisymmetric C7H16 field with a 10 % pre-ignition band
+at 700-1100 K (unicorn_amd.fields.config1, seed 1), dt = 1e-5 s, the
+reference's solver controls (100 sweeps max, relaxation 0.99, tolerance 1e-8,
+global stops).  The timed region is K consecutive steps starting from the
+seeded state (default K = 20); W warm-up steps run first on the same field and
+the state is then restored.
+
+Prints one JSON line: cell-chemistry updates/s (interior cells x steps / s),
+the roofline of the fused chemistry kernel (FP64, algorithmic operation count
+per cell-update from the reference source, timed with HIP events) and the CPU
+baseline: the reference Fortran (flang build, oracle/_ref) on a sample of the
+same workload -- the same cells' 20 steps with the sweep counts the GPU run
+found -- one process per core.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+FP64_PEAK = 78.6e12          # MI355X FP64 (vector = matrix), FLOP/s
+HBM_PEAK = 8.0e12            # B/s
+# algorithmic work of one cell-update (static counts of the reference source):
+# 12,760 basic FP64 ops + 650 transcendentals + 5,629 per species sweep
+# + 28 per Newton iteration + 18 per enthalpy sweep; 880 B in+out per cell.
+F_BASE, F_SWEEP, F_NEWTON, F_HSWEEP = 12760 + 650, 5629, 28, 18
+F_FINISH = 1300 + 28 * 3     # T(h) set-up sums and ~3 Newton iterations run in k_finish
+B_CELL = 880
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--cpu-cells", type=int, default=4000, help="CPU baseline sample size (cells)")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--li", type=int, default=1001)
+    p.add_argument("--lj", type=int, default=401)
+    return p.parse_args()
+
+
+def cpu_baseline(field0, nstars, nhs, ncells, steps):
+    """The reference Fortran on `ncells` random interior cells of the same field,
+    the same `steps` chained steps, each step forced to the GPU's sweep counts
+    (exact: every cell of the reference runs the global count).  P processes in
+    parallel, one core each; the 711 x 131 embedding grid's fixed per-step cost
+    (an empty run) is subtracted."""
+    from oracle import refrun
+    if not refrun.available():
+        return None
+    P = max(1, min(16, os.cpu_count() or 1))
+    rng = np.random.default_rng(99)
+    inter = field0.interior()
+    cells = np.sort(rng.choice(inter, size=min(ncells, len(inter)), replace=False))
+    chunks = np.array_split(cells, P)
+
+    def run_chunk(ch):
+        f = field0.take(ch)
+        t = 0.0
+        for s in range(steps):
+            r = refrun.run(f, nsteps=1, isor1=int(nstars[s]), isor2=int(nhs[s]), force=True)
+            t += r.t_chem
+            f.rho, f.y, f.tk, f.ht, f.stmf, f.stnd = r.rho, r.y, r.tk, r.ht, r.stmf, r.stnd
+        return t
+
+    empty = field0.take(inter[:1])
+    t_empty = 0.0
+    for s in range(min(steps, 3)):
+        t_empty += refrun.run(empty, nsteps=1, isor1=int(nstars[s]), isor2=int(nhs[s]), force=True).t_chem
+    t_empty_step = t_empty / min(steps, 3)
+    with ThreadPoolExecutor(P) as ex:
+        times = list(ex.map(run_chunk, chunks))
+    t = max(ti - steps * t_empty_step for ti in times)
+    return {"value": len(cells) * steps / t, "unit": "cell-updates/s", "cores": P, "kind": "reference",
+            "sample": f"{len(cells)} random interior cells of the same field, {steps} chained steps with the "
+                      f"GPU's per-step sweep counts, reference Fortran (flang -O2) in {P} processes; "
+                      f"{t_empty_step * 1e3:.1f} ms/step of 711x131 embedding-grid overhead subtracted"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the chemistry step shards by cells; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from unicorn_amd.chem import ChemStep, Controls, DeviceField
+    from unicorn_amd.fields import flame_field
+
+    f0 = flame_field(args.li, args.lj, "C7H16", 1, preignition_band=0.10)
+    ncell = (args.li - 2) * (args.lj - 2)
+    dev = DeviceField(f0)
+    init = {k: getattr(dev, k).clone() for k in ("rho", "y", "tk", "ht", "stmf", "stnd")}
+    ctx = ChemStep(f0.n)
+    ctl = Controls(dt=f0.dt)
+    for _ in range(args.warmup):
+        ctx.step(dev, ctl)
+    for k, v in init.items():
+        getattr(dev, k).copy_(v)
+    ctx.h and None
+    # the first-pass cap starts from scratch for the timed region
+    ctx.close()
+    ctx = ChemStep(f0.n)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = [ctx.step(dev, ctl) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    el = t1 - t0
+    ms_chem = sum(s.ms_rates_sweeps for s in stats)
+    launches = sum(s.chem_launches for s in stats)
+    sweeps = sum(s.cell_sweeps for s in stats)
+    flops_chem = ncell * len(stats) * (F_BASE - F_FINISH) + F_SWEEP * sweeps
+    achieved = flops_chem / (ms_chem * 1e-3)
+    nstars = [min(s.isor1, 100) for s in stats]
+    nhs = [min(s.isor2, 100) for s in stats]
+    traffic = None
+    tf = ROOT / "profiles" / "k_chem_traffic.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+    out = {
+        "metric": "cell-chem updates/s (52 sp, FP64 vs surrogate @ stated err); % FP64/HBM peak",
+        "value": ncell * args.steps / el,
+        "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded field, unicorn_amd.fields.config1)",
+        "config": {"workload": "1001x401 C7H16 field, 10% pre-ignition band, dt=1e-5 s, "
+                               f"{args.steps} chained chemistry steps, global sweep stops",
+                   "cells": ncell, "li": args.li, "lj": args.lj, "fuel": "C7H16", "dt_s": 1e-5,
+                   "parallelism": "1 GPU"},
+        "roofline": {"kernel": "k_chem", "bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK, "traffic": traffic,
+                     "avg_launch_ms": ms_chem / launches, "launches": launches,
+                     "note": "FP64 VALU; algorithmic ops of the sweeps actually executed"},
+        "hbm_algorithmic_frac": B_CELL * ncell * args.steps / el / HBM_PEAK,
+        "sweeps": {"global_species": [s.isor1 for s in stats], "global_enthalpy": [s.isor2 for s in stats],
+                   "executed_per_cell": sweeps / (ncell * args.steps),
+                   "redo_cells": [s.redo_cells for s in stats], "first_cap": [s.first_cap for s in stats]},
+        "ms_chem_kernel": ms_chem, "ms_total_device": sum(s.ms_total for s in stats),
+    }
+    if not args.no_cpu:
+        cpu = cpu_baseline(f0, nstars, nhs, args.cpu_cells, args.steps)
+        if cpu:
+            out["cpu_baseline"] = cpu
+            out["gpu_over_cpu"] = out["value"] / cpu["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -83,7 +83,9 @@ typedef struct {
     double resdc, resdh;    /* SPGSOLV residuals (spgsolv.f:1299-1416) */
     int64_t cell_sweeps;    /* species sweeps executed, summed over cells */
     int64_t redo_cells;     /* cells recomputed after the global stop was known */
-    double ms_rates_sweeps, ms_total; /* device time of the fused kernel, of the step */
+    double ms_rates_sweeps, ms_total; /* device time of the fused kernel launches, of the step */
+    int32_t chem_launches;  /* fused-kernel launches this step (first pass, continuation, redo) */
+    int32_t first_cap;      /* sweep cap of the first pass (the previous step's count) */
 } uc_stats;
 
 /* Device-resident field (all pointers are device pointers). */

@@ -148,7 +148,8 @@ static void ck_cell(const uc_scales *S, double tk, double rho, const double *fa,
     const double sournu = c1 * rho * 1.0e+04 * exp(-2.0632e+04 / tkd) / S->wm[21];
     sour[0] = sournu + c1 * rho * asst * 3.468e+03 * exp(-1.2077e+04 / tkd) / S->wm[21];
     sour[1] = 2.0 * sournu * AVNO / CMIN / S->wmstr;
-    sour[2] = c1 * rho * asst * 1.0e+04 * pow(tkd, 0.5) * exp(-1.9626e+04 / tkd) / S->wm[1];
+    /* TKD**ALST3 with ALST3 = 0.5: the flang build evaluates it as a square root */
+    sour[2] = c1 * rho * asst * 1.0e+04 * sqrt(tkd) * exp(-1.9626e+04 / tkd) / S->wm[1];
     sour[3] = F32(0.13) * c1 * asst * 1.06e+02 * pow(tkd, -0.5) * exp(-0.0 / tkd) / S->wm[4] / tbody;
     sour[4] = 0.5 * c1 * asst * 5.54e+00 * pow(tkd, -0.5) * exp(-0.0 / tkd) / S->wm[3] / tbody;
     sour[5] = 2.0 * c1 * S->rstr * CAGL * sqrt(dpst * 6.0 * SIGB * tkd / RSOOT) * rho * rho * stnd;

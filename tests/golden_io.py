@@ -1,0 +1,54 @@
+"""Loading the golden fixtures written by tests/golden/make_golden.py."""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+
+from unicorn_amd.fields import ChemField
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+CASES = ("cfg0", "c7band", "sooty")
+
+
+def load(name: str):
+    z = np.load(GOLDEN / f"{name}.npz", allow_pickle=False)
+    f = ChemField(int(z["li"]), int(z["lj"]), z["in_rho"].copy(), z["in_y"].copy(), z["in_tk"].copy(),
+                  z["in_ht"].copy(), z["in_stmf"].copy(), z["in_stnd"].copy(),
+                  z["in_iskip"].astype(np.int32), float(z["dt"]), str(z["fuel"]))
+    return f, z
+
+
+def expected(z, nsteps: int) -> dict:
+    p = f"s{nsteps}_"
+    return {k[len(p):]: z[k] for k in z.files if k.startswith(p)}
+
+
+# tolerances of the parity bar (floating point, FP64 throughout)
+REL_Y, ABS_Y = 1e-8, 1e-14      # post-step mass fractions
+REL_T = 1e-8                    # post-step temperature
+REL_RATE = 1e-10                # production rates, relative to the field's largest |rate| of that species
+
+
+def assert_state_close(got: ChemField, ref: dict, inter: np.ndarray, label=""):
+    y, yr = got.y[:, inter], ref["y"][:, inter]
+    bad = np.abs(y - yr) > REL_Y * np.abs(yr) + ABS_Y
+    assert not bad.any(), f"{label}: {bad.sum()} mass fractions off, max |dY| {np.abs(y - yr).max():.3e}"
+    dT = np.abs(got.tk[inter] - ref["tk"][inter]) / ref["tk"][inter]
+    assert dT.max() <= REL_T, f"{label}: max rel dT {dT.max():.3e}"
+    for name in ("rho", "ht"):
+        a, b = getattr(got, name)[inter], ref[name][inter]
+        d = np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+        assert d.max() <= REL_T, f"{label}: max rel d{name} {d.max():.3e}"
+    for name in ("stmf", "stnd"):
+        a, b = getattr(got, name)[inter], ref[name][inter]
+        assert np.all(np.abs(a - b) <= REL_Y * np.abs(b) + 1e-300), f"{label}: {name}"
+
+
+def assert_rates_close(qdot, qdsp_cells, ref_qdot, ref_qdsp_cells, inter, label=""):
+    sc = np.abs(ref_qdot[inter]).max()
+    d = np.abs(qdot[inter] - ref_qdot[inter])
+    assert d.max() <= REL_RATE * sc, f"{label}: QDOT max |d| {d.max():.3e} vs scale {sc:.3e}"
+    scale_k = np.maximum(np.abs(ref_qdsp_cells).max(axis=1, keepdims=True), 1e-300)
+    dk = np.abs(qdsp_cells - ref_qdsp_cells)
+    assert np.all(dk <= REL_RATE * scale_k), f"{label}: QDSP max rel-to-scale {(dk / scale_k).max():.3e}"

@@ -41,7 +41,8 @@ class UcControls(C.Structure):
 class UcStats(C.Structure):
     _fields_ = [("isor1", C.c_int32), ("isor2", C.c_int32), ("resdc", C.c_double),
                 ("resdh", C.c_double), ("cell_sweeps", C.c_int64), ("redo_cells", C.c_int64),
-                ("ms_rates_sweeps", C.c_double), ("ms_total", C.c_double)]
+                ("ms_rates_sweeps", C.c_double), ("ms_total", C.c_double),
+                ("chem_launches", C.c_int32), ("first_cap", C.c_int32)]
 
 
 class UcField(C.Structure):
@@ -124,11 +125,13 @@ class StepStats:
     redo_cells: int
     ms_rates_sweeps: float
     ms_total: float
+    chem_launches: int
+    first_cap: int
 
     @classmethod
     def of(cls, s: UcStats) -> "StepStats":
         return cls(s.isor1, s.isor2, s.resdc, s.resdh, s.cell_sweeps, s.redo_cells,
-                   s.ms_rates_sweeps, s.ms_total)
+                   s.ms_rates_sweeps, s.ms_total, s.chem_launches, s.first_cap)
 
 
 class DeviceField:

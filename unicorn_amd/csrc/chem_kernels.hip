@@ -151,10 +151,16 @@ struct ChemArgs {
   int32_t* nsw;      // [n]       sweeps executed
   double* rhsz; double* qdot; double* qdsp;
   double* rcs;       // RC slabs, G*544
-  double* rsd_part;  // [nmax][nwaves] per-wave sums of the per-sweep residual (may be null)
+  double* rsd_part;  // [isor1][nwaves] per-wave sums of the per-sweep residual (may be null)
   int32_t nmax, mode;
+  int32_t acc_from;  // residuals of sweeps >= acc_from are accumulated (when rsd_part is set)
   double dt, sor1, sor2, tolrsp;
 };
+
+// nsw word: sweeps executed, plus NSW_CAPPED when the cell stopped on the sweep
+// cap instead of at a bitwise fixed point
+constexpr int32_t NSW_CAPPED = 1 << 30;
+constexpr int32_t NSW_MASK = NSW_CAPPED - 1;
 
 __global__ void __launch_bounds__(256) k_chem(const UcParams* __restrict__ Pg, ChemArgs a) {
   UcP P = (UcP)Pg;
@@ -173,19 +179,21 @@ __global__ void __launch_bounds__(256) k_chem(const UcParams* __restrict__ Pg, C
     const bool valid = q < a.ncells;
     const int64_t c = valid ? a.cells[q] : a.cells[0];
     const FieldBuf Y = field_buf(a.y, b52, c);
-    double fa[NSP];
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) fa[k] = Y.ld(k * n);
     const double rho = a.rho[c], tk = a.tk[c], ht = a.ht[c], stmf = a.stmf[c], stnd = a.stnd[c];
     double tbody;
-    uc_rates(P, tk, rho, fa, rc, tbody);
+    {
+      double fa[NSP];
+#pragma unroll
+      for (int k = 0; k < NSP; ++k) fa[k] = Y.ld(k * n);
+      uc_rates(P, tk, rho, fa, rc, tbody);
+    }
     uc_soot(P, rc, tk, rho, stmf, stnd, tbody);
     double s[54];
 #pragma unroll
-    for (int k = 0; k < NSP; ++k) s[k] = fa[k];
+    for (int k = 0; k < NSP; ++k) s[k] = Y.ld(k * n);  // SA = FA (spgsolv.f:169-220), re-read
     s[52] = stmf;
     s[53] = stnd;
-    bool active = valid;
+    bool active = valid, fixed = false;
     int32_t nsw = 0;
     double sst2_prev = stnd;
     for (int sw = 0; sw < a.nmax; ++sw) {
@@ -200,9 +208,10 @@ __global__ void __launch_bounds__(256) k_chem(const UcParams* __restrict__ Pg, C
           if (r / 51.0 <= a.tolrsp) active = false;
         } else if (r == 0.0) {
           active = false;  // bitwise fixed point: every later sweep repeats this one
+          fixed = true;
         }
       }
-      if (a.rsd_part) {
+      if (a.rsd_part && sw >= a.acc_from) {
         const double wsum = wave_sum(r);
         if ((threadIdx.x & (WAVE - 1)) == 0) a.rsd_part[(int64_t)sw * nwaves + wave] += wsum;
       }
@@ -274,7 +283,7 @@ __global__ void __launch_bounds__(256) k_chem(const UcParams* __restrict__ Pg, C
       const double rhs1n = rho * stnd + a.dt * rc.ld(UC_SLOT_SOUR + 1) * s[21];
       SS.st(3 * n, rhs1n * a.sor1 / (rho + a.dt * rc.ld(UC_SLOT_SOUR + 5)));
     }
-    a.nsw[c] = nsw;
+    a.nsw[c] = nsw | (fixed ? 0 : NSW_CAPPED);
   }
 }
 
@@ -387,7 +396,7 @@ __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg,
     const FieldBuf SS = field_buf(a.sst, n * 8 * 4, c);
     double sst2 = SS.ld(n);
     if (a.mode != UC_MODE_PERCELL) {
-      const int32_t done = a.nsw[c];
+      const int32_t done = a.nsw[c] & NSW_MASK;
       if (a.nsweeps == done - 1) {
         sst2 = SS.ld(2 * n);
       } else {
@@ -483,6 +492,8 @@ struct uc_context {
   double* rows = nullptr;  // per-sweep residual sums (<= 1024 sweeps)
   hipEvent_t ev[4]{};
   double last_dt = -1.0;
+  int32_t last_nstar = 0;   // previous step's species sweep count (first-pass cap)
+  int blocks_per_cu = 1;
   // cell list cache
   const int32_t* mask_ptr = nullptr;
   int32_t mask_li = 0, mask_lj = 0;
@@ -529,8 +540,13 @@ int uc_create(const uc_scales* scales, int64_t max_cells, uc_context** out) {
   fill_params(ctx->sc, ctx->hp);
   ctx->max_cells = max_cells;
   // RC slabs: enough threads to fill the device a few waves deep, capped by the field
+  // one RC slab per resident thread of k_chem: fill every CU to the kernel's occupancy
   const int64_t cus = prop.multiProcessorCount;
-  ctx->G = std::max<int64_t>(256, std::min<int64_t>(((max_cells + 255) / 256) * 256, cus * 128));
+  int bpc = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_chem, 256, 0));
+  ctx->blocks_per_cu = std::max(1, bpc);
+  ctx->G = std::max<int64_t>(256, std::min<int64_t>(((max_cells + 255) / 256) * 256,
+                                                     cus * ctx->blocks_per_cu * 256));
   HIPCHK(hipMalloc(&ctx->dp, sizeof(UcParams)));
   HIPCHK(hipMemcpy(ctx->dp, &ctx->hp, sizeof(UcParams), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&ctx->rcs, sizeof(double) * SLAB_SLOTS * ctx->G));
