@@ -21,7 +21,6 @@ import argparse
 import json
 import os
 import sys
-import tempfile
 import time
 from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
@@ -112,7 +111,6 @@ def main():
         ctx.step(dev, ctl)
     for k, v in init.items():
         getattr(dev, k).copy_(v)
-    ctx.h and None
     # the first-pass cap starts from scratch for the timed region
     ctx.close()
     ctx = ChemStep(f0.n)

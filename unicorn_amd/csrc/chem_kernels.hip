@@ -154,6 +154,7 @@ struct ChemArgs {
   double* rsd_part;  // [isor1][nwaves] per-wave sums of the per-sweep residual (may be null)
   int32_t nmax, mode;
   int32_t acc_from;  // residuals of sweeps >= acc_from are accumulated (when rsd_part is set)
+  int32_t rsd_stride;  // row stride of rsd_part: waves of a full-size launch (>= this launch's)
   double dt, sor1, sor2, tolrsp;
 };
 
@@ -167,7 +168,7 @@ __global__ void __launch_bounds__(256) k_chem(const UcParams* __restrict__ Pg, C
   const int64_t G = (int64_t)gridDim.x * blockDim.x;
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int wave = (int)(tid / WAVE);
-  const int nwaves = (int)(G / WAVE);
+  const int64_t rstride = a.rsd_stride;
   RcSlab rc = wave_slab(a.rcs);
   const int64_t n = a.n;
   const int64_t b1 = n * 8, b52 = n * 8 * NSP;
@@ -213,10 +214,14 @@ __global__ void __launch_bounds__(256) k_chem(const UcParams* __restrict__ Pg, C
       }
       if (a.rsd_part && sw >= a.acc_from) {
         const double wsum = wave_sum(r);
-        if ((threadIdx.x & (WAVE - 1)) == 0) a.rsd_part[(int64_t)sw * nwaves + wave] += wsum;
+        if ((threadIdx.x & (WAVE - 1)) == 0) a.rsd_part[sw * rstride + wave] += wsum;
       }
     }
     if (!valid) continue;
+    // SST2 increment with the final species, before the closure rescales them (for
+    // completing the decided sweep count later, solvesp.f:2820-2831)
+    const double sst2_b = (rho * stnd + a.dt * rc.ld(UC_SLOT_SOUR + 1) * s[21]) * a.sor1
+                          / (rho + a.dt * rc.ld(UC_SLOT_SOUR + 5));
     // N2 closure (solvesp.f:2839-2914)
     double ftot = 0.0;
 #pragma unroll
@@ -279,10 +284,7 @@ __global__ void __launch_bounds__(256) k_chem(const UcParams* __restrict__ Pg, C
     SS.st(0, s[52]);
     SS.st(n, s[53]);
     SS.st(2 * n, sst2_prev);
-    {  // SST2 increment with the final species (for completing the count later)
-      const double rhs1n = rho * stnd + a.dt * rc.ld(UC_SLOT_SOUR + 1) * s[21];
-      SS.st(3 * n, rhs1n * a.sor1 / (rho + a.dt * rc.ld(UC_SLOT_SOUR + 5)));
-    }
+    SS.st(3 * n, sst2_b);
     a.nsw[c] = nsw | (fixed ? 0 : NSW_CAPPED);
   }
 }
@@ -557,7 +559,7 @@ int uc_create(const uc_scales* scales, int64_t max_cells, uc_context** out) {
   HIPCHK(hipMalloc(&ctx->cells, sizeof(int32_t) * max_cells));
   HIPCHK(hipMalloc(&ctx->redo, sizeof(int32_t) * max_cells));
   HIPCHK(hipMalloc(&ctx->done, max_cells));
-  ctx->rsd_part_len = (int64_t)128 * (ctx->G / WAVE);
+  ctx->rsd_part_len = (int64_t)1024 * (ctx->G / WAVE);  // isor1 <= 127, isor2 <= 1000 rows
   HIPCHK(hipMalloc(&ctx->rsd_part, sizeof(double) * ctx->rsd_part_len));
   ctx->res_part_len = 2 * ((max_cells + 255) / 256 + 1);
   HIPCHK(hipMalloc(&ctx->res_part, sizeof(double) * ctx->res_part_len));

@@ -81,6 +81,11 @@ class ChemField:
         i, j = np.meshgrid(np.arange(1, self.li - 1), np.arange(1, self.lj - 1), indexing="ij")
         return (i * self.lj + j).ravel()
 
+    def active(self) -> np.ndarray:
+        """Interior cells the chemistry advances (ISKIP < 3, ckrates.f:93)."""
+        c = self.interior()
+        return c[self.iskip[c] < 3]
+
     def copy(self) -> "ChemField":
         return ChemField(self.li, self.lj, self.rho.copy(), self.y.copy(), self.tk.copy(),
                          self.ht.copy(), self.stmf.copy(), self.stnd.copy(), self.iskip.copy(),
