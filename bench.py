@@ -20,6 +20,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -45,11 +46,27 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--cpu-cells", type=int, default=4000, help="CPU baseline sample size (cells)")
+    p.add_argument("--cpu-cells", type=int, default=16000, help="CPU baseline sample size (cells)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--li", type=int, default=1001)
     p.add_argument("--lj", type=int, default=401)
+    p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                   help="only run a plain device copy of MIB MiB (5 times): the reference "
+                        "kernel for scaling FETCH_SIZE / WRITE_SIZE counter readings")
     return p.parse_args()
+
+
+def copy_probe(mib: int):
+    """A plain copy of known size: reads and writes mib MiB per launch."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
 
 
 def cpu_baseline(field0, nstars, nhs, ncells, steps):
@@ -76,22 +93,29 @@ def cpu_baseline(field0, nstars, nhs, ncells, steps):
             f.rho, f.y, f.tk, f.ht, f.stmf, f.stnd = r.rho, r.y, r.tk, r.ht, r.stmf, r.stnd
         return t
 
+    # the embedding grid's own cost per step (loops over all 711 x 131 cells, ISKIP
+    # tests, whole-grid SOLVEH sweeps) with the same sweep counts, from a 1-cell run
     empty = field0.take(inter[:1])
-    t_empty = 0.0
-    for s in range(min(steps, 3)):
-        t_empty += refrun.run(empty, nsteps=1, isor1=int(nstars[s]), isor2=int(nhs[s]), force=True).t_chem
-    t_empty_step = t_empty / min(steps, 3)
+
+    def run_empty(s):
+        return refrun.run(empty, nsteps=1, isor1=int(nstars[s]), isor2=int(nhs[s]), force=True).t_chem
+
     with ThreadPoolExecutor(P) as ex:
+        t_empty = sum(ex.map(run_empty, range(steps)))
         times = list(ex.map(run_chunk, chunks))
-    t = max(ti - steps * t_empty_step for ti in times)
+    t = max(times) - t_empty
     return {"value": len(cells) * steps / t, "unit": "cell-updates/s", "cores": P, "kind": "reference",
             "sample": f"{len(cells)} random interior cells of the same field, {steps} chained steps with the "
-                      f"GPU's per-step sweep counts, reference Fortran (flang -O2) in {P} processes; "
-                      f"{t_empty_step * 1e3:.1f} ms/step of 711x131 embedding-grid overhead subtracted"}
+                      f"GPU's per-step sweep counts, reference Fortran (flang -O2) in {P} processes of "
+                      f"{len(chunks[0])} cells; {t_empty:.2f} s of 711x131 embedding-grid overhead "
+                      f"({steps} steps) subtracted from the slowest process's {max(times):.2f} s"}
 
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the chemistry step shards by cells; see DESIGN.md)", file=sys.stderr)
@@ -127,10 +151,11 @@ def main():
     achieved = flops_chem / (ms_chem * 1e-3)
     nstars = [min(s.isor1, 100) for s in stats]
     nhs = [min(s.isor2, 100) for s in stats]
-    traffic = None
-    tf = ROOT / "profiles" / "k_chem_traffic.json"
+    traffic = None  # HBM bytes per k_chem launch from the committed FETCH_SIZE/WRITE_SIZE runs
+    tf = ROOT / "profiles" / "k_chem_hbm_c7h16_1001x401.md"
     if tf.exists():
-        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+        m = re.search(r"^traffic_bytes_per_launch: ([0-9.eE+-]+)", tf.read_text(), re.M)
+        traffic = float(m.group(1)) if m else None
     out = {
         "metric": "cell-chem updates/s (52 sp, FP64 vs surrogate @ stated err); % FP64/HBM peak",
         "value": ncell * args.steps / el,
