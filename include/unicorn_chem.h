@@ -81,11 +81,13 @@ typedef struct {
 typedef struct {
     int32_t isor1, isor2;   /* sweeps used, reference convention (max+1 = not converged) */
     double resdc, resdh;    /* SPGSOLV residuals (spgsolv.f:1299-1416) */
-    int64_t cell_sweeps;    /* species sweeps executed, summed over cells */
-    int64_t redo_cells;     /* cells recomputed after the global stop was known */
-    double ms_rates_sweeps, ms_total; /* device time of the fused kernel launches, of the step */
-    int32_t chem_launches;  /* fused-kernel launches this step (first pass, continuation, redo) */
-    int32_t first_cap;      /* sweep cap of the first pass (the previous step's count) */
+    int64_t cell_sweeps;    /* species sweeps executed, summed over cells and launches */
+    int64_t redo_cells;     /* cells re-run because the decided count fell before the
+                               snapshot window of the last segment */
+    double ms_rates, ms_sweeps, ms_sources, ms_total; /* device time: rate coefficients,
+                               sweep segments, sources; whole step */
+    int32_t sweep_launches; /* sweep-segment launches this step */
+    int32_t first_segment;  /* sweeps in the first segment (the previous count minus one) */
 } uc_stats;
 
 /* Device-resident field (all pointers are device pointers). */
@@ -110,9 +112,10 @@ const char *uc_last_error(void);
 int uc_scales_from_card(double vref, double tref, double pref, double rtin, double rtot,
                         double fo2in, int32_t igrav, uc_scales *out);
 
-/* A context for fields of up to max_cells cells on the current HIP device.
- * It owns the scratch that the fused kernel needs (per-thread RC slabs,
- * sweep-residual partial sums); it allocates with hipMalloc. */
+/* A context for fields of up to max_cells cells (at most 4,000,000) on the
+ * current HIP device.  It owns the per-cell device memory of a step (RC slabs,
+ * saved iterates and snapshots, sources; about 9.7 KB per cell) and the
+ * sweep-residual partial sums; it allocates with hipMalloc. */
 int uc_create(const uc_scales *scales, int64_t max_cells, uc_context **out);
 int uc_destroy(uc_context *ctx);
 

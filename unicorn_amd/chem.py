@@ -41,8 +41,8 @@ class UcControls(C.Structure):
 class UcStats(C.Structure):
     _fields_ = [("isor1", C.c_int32), ("isor2", C.c_int32), ("resdc", C.c_double),
                 ("resdh", C.c_double), ("cell_sweeps", C.c_int64), ("redo_cells", C.c_int64),
-                ("ms_rates_sweeps", C.c_double), ("ms_total", C.c_double),
-                ("chem_launches", C.c_int32), ("first_cap", C.c_int32)]
+                ("ms_rates", C.c_double), ("ms_sweeps", C.c_double), ("ms_sources", C.c_double),
+                ("ms_total", C.c_double), ("sweep_launches", C.c_int32), ("first_segment", C.c_int32)]
 
 
 class UcField(C.Structure):
@@ -123,15 +123,16 @@ class StepStats:
     resdh: float
     cell_sweeps: int
     redo_cells: int
-    ms_rates_sweeps: float
+    ms_rates: float
+    ms_sweeps: float
+    ms_sources: float
     ms_total: float
-    chem_launches: int
-    first_cap: int
+    sweep_launches: int
+    first_segment: int
 
     @classmethod
     def of(cls, s: UcStats) -> "StepStats":
-        return cls(s.isor1, s.isor2, s.resdc, s.resdh, s.cell_sweeps, s.redo_cells,
-                   s.ms_rates_sweeps, s.ms_total, s.chem_launches, s.first_cap)
+        return cls(*(getattr(s, n) for n, _ in UcStats._fields_))
 
 
 class DeviceField:

@@ -41,6 +41,7 @@ namespace {
 constexpr int NSP = 52;
 constexpr int NRX = 544;
 constexpr int WAVE = 64;
+constexpr int64_t UC_MAX_CELLS = 4000000;  // 54 doubles x cells < 2^31 bytes
 
 struct UcParams {
   double tstr, alstr, vstr, rstr, wmstr, hstr, gasc, pback, beta1;
@@ -142,151 +143,261 @@ __global__ void k_clip(double* __restrict__ y, int64_t n) {
   y[(NSP - 1) * n + c] = 1.0 - tmass;
 }
 
-struct ChemArgs {
-  const double* rho; const double* y; const double* tk; const double* ht;
-  const double* stmf; const double* stnd;
+// nsw word of a cell: species sweeps executed, plus NSW_FIXED when the cell
+// stopped at a bitwise fixed point (a sweep that changed nothing; in the
+// per-cell mode: when it met its own tolerance)
+constexpr int32_t NSW_FIXED = 1 << 30;
+constexpr int32_t NSW_MASK = NSW_FIXED - 1;
+
+// Per-cell buffers are indexed by the cell's position q in the cell list;
+// species-major arrays have a row stride of npad (the list length rounded up
+// to whole waves).  The slab of cell q is tile q/64, lane q%64 -- thread q of a
+// one-thread-per-cell launch, so the tile is the launch's wave index.
+
+struct RatesArgs {
+  const double* rho; const double* y; const double* tk; const double* stmf; const double* stnd;
   const int32_t* cells; int32_t ncells; int64_t n;
-  double* sa;        // [52][n]   SA* after closure
-  double* sst;       // [4][n]    SST1, SST2, SST2 before the last sweep, SST2 increment b
-  int32_t* nsw;      // [n]       sweeps executed
-  double* rhsz; double* qdot; double* qdsp;
-  double* rcs;       // RC slabs, G*544
-  double* rsd_part;  // [isor1][nwaves] per-wave sums of the per-sweep residual (may be null)
-  int32_t nmax, mode;
-  int32_t acc_from;  // residuals of sweeps >= acc_from are accumulated (when rsd_part is set)
-  int32_t rsd_stride;  // row stride of rsd_part: waves of a full-size launch (>= this launch's)
+  double* slabs;
+};
+
+// rate coefficients, rho*Y_k and soot sources of every cell into its slab
+// (ckrates.f:92-1352)
+__global__ void __launch_bounds__(256) k_rates(const UcParams* __restrict__ Pg, RatesArgs a) {
+  UcP P = (UcP)Pg;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c = a.cells[q < a.ncells ? q : a.ncells - 1];  // tail lanes redo the last cell
+  RcSlab rc = wave_slab(a.slabs);
+  const FieldBuf Y = field_buf(a.y, a.n * 8 * NSP, c);
+  const double rho = a.rho[c], tk = a.tk[c];
+  double tbody;
+  {
+    double fa[NSP];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) fa[k] = Y.ld(k * a.n);
+    uc_rates(P, tk, rho, fa, rc, tbody);
+  }
+  uc_soot(P, rc, tk, rho, a.stmf[c], a.stnd[c], tbody);
+}
+
+struct SweepArgs {
+  const double* rho; const double* y; const double* stmf; const double* stnd;
+  const int32_t* cells; int32_t ncells; int64_t n; int64_t npad;
+  double* slabs;
+  double* st;          // [54][npad] iterates where each cell stopped
+  double* snap;        // [nsnap][54][npad] iterates after sweeps snap_from.. (may be null)
+  double* sst2p;       // [npad] SST2 before a fixed cell's last sweep
+  int32_t* nsw;        // [npad] NSW word
+  double* rsd_part;    // [isor1][rsd_stride] per-wave residual sums of each sweep (may be null)
+  int64_t rsd_stride;
+  unsigned long long* swcount;  // sweeps executed (+=)
+  int32_t* redo_count;          // cells re-run (+=, redo launches)
+  int32_t start, count;  // sweeps start+1 .. start+count (counts are 1-based)
+  int32_t snap_from;     // first sweep count that is snapshotted
+  int32_t nsnap;
+  int32_t redo;          // > 0: re-run from scratch, to this count, the cells whose iterates
+                         // at that count are neither in st nor in a snapshot
+  int32_t mode;
   double dt, sor1, sor2, tolrsp;
 };
 
-// nsw word: sweeps executed, plus NSW_CAPPED when the cell stopped on the sweep
-// cap instead of at a bitwise fixed point
-constexpr int32_t NSW_CAPPED = 1 << 30;
-constexpr int32_t NSW_MASK = NSW_CAPPED - 1;
-
-__global__ void __launch_bounds__(256) k_chem(const UcParams* __restrict__ Pg, ChemArgs a) {
+// A segment of Gauss-Seidel sweeps (solvesp.f:87-2831) for every cell that has
+// not stopped: from Y (start = 0) or from the saved iterates, with the slab's
+// RC.  Sweeps are deterministic functions of the iterates and the slab, so a
+// segment continues the previous one bit for bit.
+// One wave per block; the block's dynamic LDS (UC_SWEEP_LDS doubles per lane) is
+// the sweep's second-level cache of slab values.
+__global__ void __launch_bounds__(WAVE) k_sweep(const UcParams* __restrict__ Pg, SweepArgs a) {
+  extern __shared__ double uc_lds[];
+  double* const lds = uc_lds + threadIdx.x;
   UcP P = (UcP)Pg;
-  const int64_t G = (int64_t)gridDim.x * blockDim.x;
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int wave = (int)(tid / WAVE);
-  const int64_t rstride = a.rsd_stride;
-  RcSlab rc = wave_slab(a.rcs);
-  const int64_t n = a.n;
-  const int64_t b1 = n * 8, b52 = n * 8 * NSP;
-  // every lane of a wave runs the same number of outer iterations
-  const int64_t iters = (a.ncells + G - 1) / G;
-  for (int64_t it = 0; it < iters; ++it) {
-    asm volatile("" : "+s"(P));  // parameter loads stay inside the cell loop
-    const int64_t q = it * G + tid;
-    const bool valid = q < a.ncells;
-    const int64_t c = valid ? a.cells[q] : a.cells[0];
-    const FieldBuf Y = field_buf(a.y, b52, c);
-    const double rho = a.rho[c], tk = a.tk[c], ht = a.ht[c], stmf = a.stmf[c], stnd = a.stnd[c];
-    double tbody;
-    {
-      double fa[NSP];
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = q < a.ncells;
+  const int64_t qq = valid ? q : a.ncells - 1;
+  const int64_t wave = q / WAVE;
+  const int32_t w0 = a.nsw[qq];
+  bool active;
+  if (a.redo > 0) {
+    const int32_t m = w0 & NSW_MASK;
+    const bool fx = (w0 & NSW_FIXED) != 0;
+    active = valid && !(fx && m - 1 <= a.redo) && !(!fx && m == a.redo);
+  } else {
+    active = valid && (a.start == 0 || (w0 & NSW_FIXED) == 0);
+  }
+  if (!__any(active)) return;  // this wave's residual rows stay zero
+  const bool was_active = active;
+  if (a.redo > 0) {
+    const unsigned long long b = __ballot(active);
+    if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(a.redo_count, (int32_t)__popcll(b));
+  }
+  RcSlab rc = wave_slab(a.slabs);
+  const int64_t c = a.cells[qq];
+  const double rho = a.rho[c], stmf = a.stmf[c], stnd = a.stnd[c];
+  double s[54];
+  int32_t nsw;
+  if (a.start == 0) {  // SA = FA (spgsolv.f:169-220), soot from the field
+    const FieldBuf Y = field_buf(a.y, a.n * 8 * NSP, c);
 #pragma unroll
-      for (int k = 0; k < NSP; ++k) fa[k] = Y.ld(k * n);
-      uc_rates(P, tk, rho, fa, rc, tbody);
-    }
-    uc_soot(P, rc, tk, rho, stmf, stnd, tbody);
-    double s[54];
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) s[k] = Y.ld(k * n);  // SA = FA (spgsolv.f:169-220), re-read
+    for (int k = 0; k < NSP; ++k) s[k] = Y.ld(k * a.n);
     s[52] = stmf;
     s[53] = stnd;
-    bool active = valid, fixed = false;
-    int32_t nsw = 0;
-    double sst2_prev = stnd;
-    for (int sw = 0; sw < a.nmax; ++sw) {
-      if (!__any(active)) break;
-      double r = 0.0;
-      if (active) {
-        sst2_prev = s[53];
-        double d2;
-        r = uc_sweep(P, rc, s, rho, stmf, stnd, a.dt, a.sor1, a.sor2, &d2);
-        ++nsw;
-        if (a.mode == UC_MODE_PERCELL) {
-          if (r / 51.0 <= a.tolrsp) active = false;
-        } else if (r == 0.0) {
-          active = false;  // bitwise fixed point: every later sweep repeats this one
-          fixed = true;
-        }
-      }
-      if (a.rsd_part && sw >= a.acc_from) {
-        const double wsum = wave_sum(r);
-        if ((threadIdx.x & (WAVE - 1)) == 0) a.rsd_part[sw * rstride + wave] += wsum;
-      }
-    }
-    if (!valid) continue;
-    // SST2 increment with the final species, before the closure rescales them (for
-    // completing the decided sweep count later, solvesp.f:2820-2831)
-    const double sst2_b = (rho * stnd + a.dt * rc.ld(UC_SLOT_SOUR + 1) * s[21]) * a.sor1
-                          / (rho + a.dt * rc.ld(UC_SLOT_SOUR + 5));
-    // N2 closure (solvesp.f:2839-2914)
-    double ftot = 0.0;
+    nsw = 0;
+  } else {
+    const FieldBuf ST = field_buf(a.st, a.npad * 8 * 54, qq);
 #pragma unroll
-    for (int k = 0; k < 51; ++k) ftot += s[k];
-    ftot += s[52];
-    s[51] = 1.0 - ftot;
-    if (ftot > 1.0) {
-#pragma unroll
-      for (int k = 0; k < 51; ++k) s[k] = s[k] / ftot;
-      s[52] = s[52] / ftot;
-      s[51] = 0.0;
-    }
-    // SOLVEH sources with the final iterates (solveh.f:34-1142)
-    const double T = tk * P->tstr;
-    const double T2 = T * T, T3 = T * T2, T4 = T * T3, T5 = T * T4;
-    const int ip = T > 1000.0 ? 7 : 0;
-    const double hconvr = P->gasc / P->wmstr / P->hstr;
-    double tmole = 0.0;
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) tmole += s[k] * P->rwm[k];
-    const double tkdi = 1000.0 / T, tkdis = tkdi * tkdi;
-    const double p1 = (double)6.6334f + T * (-3.5686e-03 + T * (1.6682e-08 + T * (2.5611e-10 - 2.6558e-14 * T)));
-    const double p2 = 1.8741e+01 - tkdi * (1.2131e+02 - 2.7350e+02 * tkdi + 1.9405e+02 * tkdis
-                                           - 5.6310e+01 * tkdis * tkdi + (double)5.81690f * tkdis * tkdis);
-    const double p3 = T <= 750.0
-        ? (double)4.7869f + T * (-6.953e-02 + T * (2.95775e-04 + T * (-4.25732e-07 + 2.02894e-10 * T)))
-        : 1.009e+01 + T * (-1.183e-02 + T * (4.7753e-06 + T * (-5.87209e-10 - 2.5334e-14 * T)));
-    const double p4 = -(double)0.23093f - tkdi * ((double)1.12390f - (double)9.41530f * tkdi
-                      + (double)2.99880f * tkdis - (double)0.51382f * tkdis * tkdi + 1.86840e-05 * tkdis * tkdis);
-    const double plankc = (p1 * s[12] * P->rwm[12] + p2 * s[9] * P->rwm[9] + p3 * s[8] * P->rwm[8]
-                           + p4 * s[5] * P->rwm[5]) / tmole;
-    const double crad = P->alstr / P->rstr / P->vstr / P->vstr / P->vstr;
-    const double qrad = crad * (double)0.4f * (4.0 * 5.669e-08 * plankc * (pow(T, 4.0) - 8.1e+09)
-                                              + 3.334e-04 * stmf * (rho / 1.9e+03) * T5);
-    double rhsz = -qrad * a.dt;
-    const double rconv = P->vstr / P->wmstr / P->alstr * 1.0e-06;
-    double prod = 0.0, hrr = 0.0;
-    const FieldBuf QD = field_buf(a.qdsp, b52, c);
-    const bool hiT = ip != 0;
-    uc_prd(P, rc, s, [&](UcP Q, int k, double pr) {
-      // per-species enthalpy HENTH (solveh.f:39-57) with this cell's NASA range
-      double pk[6];
-#pragma unroll
-      for (int i = 0; i < 6; ++i) pk[i] = hiT ? Q->nasa[k][7 + i] : Q->nasa[k][i];
-      const double hssp = (pk[0] * T + pk[1] * T2 / 2.0 + pk[2] * T3 / 3.0 + pk[3] * T4 / 4.0
-                           + pk[4] * T5 / 5.0) * Q->rwm[k] * hconvr;
-      const double henth = hssp * Q->wmstr * Q->hstr + pk[5] * Q->rwm[k] * Q->gasc;
-      const double prd = Q->wm[k] * Q->hfo[k] * pr;
-      prod += prd;
-      hrr += prd * henth / Q->hfo[k];
-      QD.st(k * n, -rconv * prd / Q->hfo[k]);
-    });
-    rhsz = rhsz + rho * ht - a.dt * prod;
-    a.rhsz[c] = rhsz;
-    a.qdot[c] = -hrr * rconv;
-    const FieldBuf SA = field_buf(a.sa, b52, c);
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) SA.st(k * n, s[k]);
-    const FieldBuf SS = field_buf(a.sst, 4 * b1, c);
-    SS.st(0, s[52]);
-    SS.st(n, s[53]);
-    SS.st(2 * n, sst2_prev);
-    SS.st(3 * n, sst2_b);
-    a.nsw[c] = nsw | (fixed ? 0 : NSW_CAPPED);
+    for (int k = 0; k < 54; ++k) s[k] = ST.ld(k * a.npad);
+    nsw = w0 & NSW_MASK;
   }
+  bool fixed = false;
+  double sst2_prev = 0.0;
+  uint32_t executed = 0;
+  for (int32_t sw = a.start; sw < a.start + a.count; ++sw) {
+    if (!__any(active)) break;
+    double r = 0.0;
+    if (active) {
+      sst2_prev = s[53];
+      double d2;
+      r = uc_sweep(P, rc, lds, s, rho, stmf, stnd, a.dt, a.sor1, a.sor2, &d2);
+      ++nsw;
+      ++executed;
+      if (a.mode == UC_MODE_PERCELL) {
+        if (r / 51.0 <= a.tolrsp) { active = false; fixed = true; }
+      } else if (r == 0.0) {
+        active = false;  // every later sweep repeats this one exactly
+        fixed = true;
+      }
+      if (a.snap && sw + 1 >= a.snap_from) {
+        const FieldBuf SN = field_buf(a.snap + (int64_t)(sw + 1 - a.snap_from) * 54 * a.npad,
+                                      a.npad * 8 * 54, qq);
+#pragma unroll
+        for (int k = 0; k < 54; ++k) SN.st(k * a.npad, s[k]);
+      }
+    }
+    if (a.rsd_part) {
+      const double ws = wave_sum(r);
+      if ((threadIdx.x & (WAVE - 1)) == 0) a.rsd_part[(int64_t)sw * a.rsd_stride + wave] = ws;
+    }
+  }
+  {
+    unsigned long long e = executed;
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, WAVE);
+    if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(a.swcount, e);
+  }
+  if (!was_active) return;
+  const FieldBuf ST = field_buf(a.st, a.npad * 8 * 54, q);
+#pragma unroll
+  for (int k = 0; k < 54; ++k) ST.st(k * a.npad, s[k]);
+  a.nsw[q] = nsw | (fixed ? NSW_FIXED : 0);
+  if (fixed) a.sst2p[q] = sst2_prev;
+}
+
+struct SourcesArgs {
+  const double* rho; const double* tk; const double* ht; const double* stmf; const double* stnd;
+  const int32_t* cells; int32_t ncells; int64_t n; int64_t npad;
+  const double* slabs; const double* st; const double* snap; const double* sst2p; const int32_t* nsw;
+  int32_t nstar;        // decided sweep count (global mode)
+  int32_t snap_from;    // sweep count of snapshot 0 of the last segment
+  int32_t nsnap;
+  int32_t mode;
+  double dt, sor1, sor2;
+  double* sa;           // [52][npad] species after the closure
+  double* sst;          // [2][npad]  SST1, SST2
+  double* rhsz;         // [npad]
+  double* qdot; double* qdsp;  // field arrays (solveh.f:1137-1142)
+};
+
+// Iterates at the decided count, SST2 completion, N2 closure (solvesp.f:2839-2914)
+// and the SOLVEH sources: radiation, RA/PRD, QDOT, QDSP, RHSZ (solveh.f:34-1142).
+__global__ void __launch_bounds__(256) k_sources(const UcParams* __restrict__ Pg, SourcesArgs a) {
+  UcP P = (UcP)Pg;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.ncells) return;
+  RcSlab rc = wave_slab((double*)a.slabs);
+  const int64_t c = a.cells[q], n = a.n;
+  const double rho = a.rho[c], tk = a.tk[c], ht = a.ht[c], stmf = a.stmf[c], stnd = a.stnd[c];
+  const int32_t w = a.nsw[q];
+  const int32_t m = w & NSW_MASK;
+  const bool fx = (w & NSW_FIXED) != 0;
+  const bool global = a.mode != UC_MODE_PERCELL;
+  const bool use_st = !global || (fx && m - 1 <= a.nstar) || (!fx && m == a.nstar);
+  const FieldBuf SRC = use_st ? field_buf(a.st, a.npad * 8 * 54, q)
+                              : field_buf(a.snap + (int64_t)(a.nstar - a.snap_from) * 54 * a.npad,
+                                          a.npad * 8 * 54, q);
+  double s[54];
+#pragma unroll
+  for (int k = 0; k < 54; ++k) s[k] = SRC.ld(k * a.npad);
+  if (global && fx && m - 1 <= a.nstar) {
+    // a fixed cell's species repeat; only SST2 keeps relaxing (solvesp.f:2820-2831)
+    if (a.nstar == m - 1) {
+      s[53] = a.sst2p[q];
+    } else {
+      const double b = (rho * stnd + a.dt * rc.ld(UC_SLOT_SOUR + 1) * s[21]) * a.sor1
+                       / (rho + a.dt * rc.ld(UC_SLOT_SOUR + 5));
+      for (int32_t i = m; i < a.nstar; ++i) {
+        const double fnew = a.sor2 * s[53] + b;
+        s[53] = fnew < 0.0 ? 0.0 : fnew;
+      }
+    }
+  }
+  // N2 closure (solvesp.f:2839-2914)
+  double ftot = 0.0;
+#pragma unroll
+  for (int k = 0; k < 51; ++k) ftot += s[k];
+  ftot += s[52];
+  s[51] = 1.0 - ftot;
+  if (ftot > 1.0) {
+#pragma unroll
+    for (int k = 0; k < 51; ++k) s[k] = s[k] / ftot;
+    s[52] = s[52] / ftot;
+    s[51] = 0.0;
+  }
+  // SOLVEH sources with the final iterates (solveh.f:34-1142)
+  const double T = tk * P->tstr;
+  const double T2 = T * T, T3 = T * T2, T4 = T * T3, T5 = T * T4;
+  const int ip = T > 1000.0 ? 7 : 0;
+  const double hconvr = P->gasc / P->wmstr / P->hstr;
+  double tmole = 0.0;
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) tmole += s[k] * P->rwm[k];
+  const double tkdi = 1000.0 / T, tkdis = tkdi * tkdi;
+  const double p1 = (double)6.6334f + T * (-3.5686e-03 + T * (1.6682e-08 + T * (2.5611e-10 - 2.6558e-14 * T)));
+  const double p2 = 1.8741e+01 - tkdi * (1.2131e+02 - 2.7350e+02 * tkdi + 1.9405e+02 * tkdis
+                                         - 5.6310e+01 * tkdis * tkdi + (double)5.81690f * tkdis * tkdis);
+  const double p3 = T <= 750.0
+      ? (double)4.7869f + T * (-6.953e-02 + T * (2.95775e-04 + T * (-4.25732e-07 + 2.02894e-10 * T)))
+      : 1.009e+01 + T * (-1.183e-02 + T * (4.7753e-06 + T * (-5.87209e-10 - 2.5334e-14 * T)));
+  const double p4 = -(double)0.23093f - tkdi * ((double)1.12390f - (double)9.41530f * tkdi
+                    + (double)2.99880f * tkdis - (double)0.51382f * tkdis * tkdi + 1.86840e-05 * tkdis * tkdis);
+  const double plankc = (p1 * s[12] * P->rwm[12] + p2 * s[9] * P->rwm[9] + p3 * s[8] * P->rwm[8]
+                         + p4 * s[5] * P->rwm[5]) / tmole;
+  const double crad = P->alstr / P->rstr / P->vstr / P->vstr / P->vstr;
+  const double qrad = crad * (double)0.4f * (4.0 * 5.669e-08 * plankc * (pow(T, 4.0) - 8.1e+09)
+                                            + 3.334e-04 * stmf * (rho / 1.9e+03) * T5);
+  double rhsz = -qrad * a.dt;
+  const double rconv = P->vstr / P->wmstr / P->alstr * 1.0e-06;
+  double prod = 0.0, hrr = 0.0;
+  const FieldBuf QD = field_buf(a.qdsp, n * 8 * NSP, c);
+  const bool hiT = ip != 0;
+  uc_prd(P, rc, s, [&](UcP Q, int k, double pr) {
+    // per-species enthalpy HENTH (solveh.f:39-57) with this cell's NASA range
+    double pk[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) pk[i] = hiT ? Q->nasa[k][7 + i] : Q->nasa[k][i];
+    const double hssp = (pk[0] * T + pk[1] * T2 / 2.0 + pk[2] * T3 / 3.0 + pk[3] * T4 / 4.0
+                         + pk[4] * T5 / 5.0) * Q->rwm[k] * hconvr;
+    const double henth = hssp * Q->wmstr * Q->hstr + pk[5] * Q->rwm[k] * Q->gasc;
+    const double prd = Q->wm[k] * Q->hfo[k] * pr;
+    prod += prd;
+    hrr += prd * henth / Q->hfo[k];
+    QD.st(k * n, -rconv * prd / Q->hfo[k]);
+  });
+  rhsz = rhsz + rho * ht - a.dt * prod;
+  a.rhsz[q] = rhsz;
+  a.qdot[c] = -hrr * rconv;
+  const FieldBuf SA = field_buf(a.sa, a.npad * 8 * NSP, q);
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) SA.st(k * a.npad, s[k]);
+  a.sst[q] = s[52];
+  a.sst[a.npad + q] = s[53];
 }
 
 struct HresArgs {
@@ -305,9 +416,10 @@ __global__ void __launch_bounds__(256) k_hres(HresArgs a) {
   for (int64_t it = 0; it < iters; ++it) {
     const int64_t q = it * G + tid;
     const bool valid = q < a.ncells;
-    const int64_t c = valid ? a.cells[q] : a.cells[0];
+    const int64_t qq = valid ? q : 0;
+    const int64_t c = a.cells[qq];
     double fpz = a.ht[c];
-    const double inc = a.rhsz[c] * a.zsor1 / a.rho[c];
+    const double inc = a.rhsz[qq] * a.zsor1 / a.rho[c];
     bool active = valid;
     for (int sw = 0; sw < a.nmax; ++sw) {
       if (!__any(active)) break;
@@ -326,12 +438,11 @@ __global__ void __launch_bounds__(256) k_hres(HresArgs a) {
 
 struct FinishArgs {
   double* rho; double* y; double* tk; double* ht; double* stmf; double* stnd;
-  const double* sa; const double* sst; const int32_t* nsw; const double* rhsz;
-  const int32_t* cells; int32_t ncells; int64_t n;
-  int32_t nsweeps;   // decided species sweep count (global mode)
+  const double* sa; const double* sst; const double* rhsz;  // [52][npad], [2][npad], [npad]
+  const int32_t* cells; int32_t ncells; int64_t n; int64_t npad;
   int32_t nh;        // decided enthalpy sweep count (global mode) / cap (per-cell)
   int32_t mode;
-  double zsor1, zsor2, tolrh, sor2;
+  double zsor1, zsor2, tolrh;
   double* res_part;  // [2][nblocks] RESDC, RESDH partial sums
 };
 
@@ -341,14 +452,14 @@ __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg,
   double rc_ = 0.0, rh_ = 0.0;
   if (q < a.ncells) {
     const int64_t c = a.cells[q], n = a.n;
-    const FieldBuf SA = field_buf(a.sa, n * 8 * NSP, c), Y = field_buf(a.y, n * 8 * NSP, c);
+    const FieldBuf SA = field_buf(a.sa, a.npad * 8 * NSP, q), Y = field_buf(a.y, n * 8 * NSP, c);
     double s[NSP];
 #pragma unroll
-    for (int k = 0; k < NSP; ++k) s[k] = SA.ld(k * n);
+    for (int k = 0; k < NSP; ++k) s[k] = SA.ld(k * a.npad);
     const double rho = a.rho[c], ht = a.ht[c];
     // enthalpy relaxation (solveh.f:1147-1167), zero transport: PPP = RHO
     double fpz = ht;
-    const double inc = a.rhsz[c] * a.zsor1 / rho;
+    const double inc = a.rhsz[q] * a.zsor1 / rho;
     for (int sw = 0; sw < a.nh; ++sw) {
       const double fnew = a.zsor2 * fpz + inc;
       const double d = fabs(fpz - fnew);
@@ -394,23 +505,8 @@ __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg,
     if (tkd > 5000.0) tkd = 5000.0;
     if (tkd < P->tstr - 1.0) tkd = P->tstr - 1.0;
     const double tnew = tkd / P->tstr;
-    // soot number density: complete the decided sweep count (solvesp.f:2820-2831)
-    const FieldBuf SS = field_buf(a.sst, n * 8 * 4, c);
-    double sst2 = SS.ld(n);
-    if (a.mode != UC_MODE_PERCELL) {
-      const int32_t done = a.nsw[c] & NSW_MASK;
-      if (a.nsweeps == done - 1) {
-        sst2 = SS.ld(2 * n);
-      } else {
-        const double b = SS.ld(3 * n);
-        for (int sw = done; sw < a.nsweeps; ++sw) {
-          const double fnew = a.sor2 * sst2 + b;
-          sst2 = fnew < 0.0 ? 0.0 : fnew;
-        }
-      }
-    }
     // SPGSOLV epilogue (spgsolv.f:1299-1416)
-    const double sst1 = SS.ld(0);
+    const double sst1 = a.sst[q], sst2 = a.sst[a.npad + q];
     rh_ = fabs(fpz - ht);
     rc_ = fabs(sst1 - a.stmf[c]);
     double rbar = 0.0;
@@ -477,30 +573,33 @@ struct uc_context {
   UcParams hp{};
   UcParams* dp = nullptr;
   int64_t max_cells = 0;
+  int64_t max_pad = 0;     // max_cells rounded up to whole waves
   int device = 0;
-  int64_t G = 0;           // k_chem threads (RC slabs)
-  double* rcs = nullptr;
-  double* sa = nullptr;
-  double* sst = nullptr;
-  double* rhsz = nullptr;
-  int32_t* nsw = nullptr;
+  int32_t nsnap = 0;       // snapshot slots per cell
+  double* slabs = nullptr; // [max_pad/64][SLAB_SLOTS][64]
+  double* st = nullptr;    // [54][max_pad]
+  double* snap = nullptr;  // [nsnap][54][max_pad]
+  double* sst2p = nullptr; // [max_pad]
+  double* sa = nullptr;    // [52][max_pad]
+  double* sst = nullptr;   // [2][max_pad]
+  double* rhsz = nullptr;  // [max_pad]
+  int32_t* nsw = nullptr;  // [max_pad]
   int32_t* cells = nullptr;
-  int32_t* redo = nullptr;
   uint8_t* done = nullptr;
-  double* rsd_part = nullptr;
+  double* rsd_part = nullptr;  // per-sweep, per-wave residual sums
   int64_t rsd_part_len = 0;
+  int64_t hres_threads = 0;    // k_hres grid (persistent)
   double* res_part = nullptr;
   int64_t res_part_len = 0;
-  double* rows = nullptr;  // per-sweep residual sums (<= 1024 sweeps)
-  hipEvent_t ev[4]{};
+  double* rows = nullptr;      // per-sweep residual sums (<= 1024 sweeps)
+  unsigned long long* counters = nullptr;  // [0] sweeps executed, [1] (int) cells re-run
+  hipEvent_t ev[8]{};
   double last_dt = -1.0;
-  int32_t last_nstar = 0;   // previous step's species sweep count (first-pass cap)
-  int blocks_per_cu = 1;
+  int32_t last_nstar = 0;  // previous step's species sweep count (first segment length)
   // cell list cache
   const int32_t* mask_ptr = nullptr;
   int32_t mask_li = 0, mask_lj = 0;
   std::vector<int32_t> cell_list;
-  std::vector<int32_t> prev_nsw;
 };
 
 namespace {
@@ -529,6 +628,9 @@ int uc_scales_from_card(double vref, double tref, double pref, double rtin, doub
 
 int uc_create(const uc_scales* scales, int64_t max_cells, uc_context** out) {
   if (!scales || !out || max_cells < 9) return fail(UC_E_ARG, "uc_create: bad arguments");
+  // buffer instructions address each per-cell array with 32-bit byte offsets
+  if (max_cells > UC_MAX_CELLS)
+    return fail(UC_E_ARG, "uc_create: max_cells above " + std::to_string(UC_MAX_CELLS));
   *out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(UC_E_NOGPU, "uc_create: no HIP device");
@@ -541,29 +643,31 @@ int uc_create(const uc_scales* scales, int64_t max_cells, uc_context** out) {
   ctx->sc = *scales;
   fill_params(ctx->sc, ctx->hp);
   ctx->max_cells = max_cells;
-  // RC slabs: enough threads to fill the device a few waves deep, capped by the field
-  // one RC slab per resident thread of k_chem: fill every CU to the kernel's occupancy
-  const int64_t cus = prop.multiProcessorCount;
-  int bpc = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_chem, 256, 0));
-  ctx->blocks_per_cu = std::max(1, bpc);
-  ctx->G = std::max<int64_t>(256, std::min<int64_t>(((max_cells + 255) / 256) * 256,
-                                                     cus * ctx->blocks_per_cu * 256));
+  ctx->max_pad = (max_cells + WAVE - 1) / WAVE * WAVE;
+  ctx->nsnap = 8;
+  const int64_t mp = ctx->max_pad;
   HIPCHK(hipMalloc(&ctx->dp, sizeof(UcParams)));
   HIPCHK(hipMemcpy(ctx->dp, &ctx->hp, sizeof(UcParams), hipMemcpyHostToDevice));
-  HIPCHK(hipMalloc(&ctx->rcs, sizeof(double) * SLAB_SLOTS * ctx->G));
-  HIPCHK(hipMalloc(&ctx->sa, sizeof(double) * NSP * max_cells));
-  HIPCHK(hipMalloc(&ctx->sst, sizeof(double) * 4 * max_cells));
-  HIPCHK(hipMalloc(&ctx->rhsz, sizeof(double) * max_cells));
-  HIPCHK(hipMalloc(&ctx->nsw, sizeof(int32_t) * max_cells));
+  HIPCHK(hipMalloc(&ctx->slabs, sizeof(double) * SLAB_SLOTS * mp));
+  HIPCHK(hipMalloc(&ctx->st, sizeof(double) * 54 * mp));
+  HIPCHK(hipMalloc(&ctx->snap, sizeof(double) * 54 * mp * ctx->nsnap));
+  HIPCHK(hipMalloc(&ctx->sst2p, sizeof(double) * mp));
+  HIPCHK(hipMalloc(&ctx->sa, sizeof(double) * NSP * mp));
+  HIPCHK(hipMalloc(&ctx->sst, sizeof(double) * 2 * mp));
+  HIPCHK(hipMalloc(&ctx->rhsz, sizeof(double) * mp));
+  HIPCHK(hipMalloc(&ctx->nsw, sizeof(int32_t) * mp));
   HIPCHK(hipMalloc(&ctx->cells, sizeof(int32_t) * max_cells));
-  HIPCHK(hipMalloc(&ctx->redo, sizeof(int32_t) * max_cells));
   HIPCHK(hipMalloc(&ctx->done, max_cells));
-  ctx->rsd_part_len = (int64_t)1024 * (ctx->G / WAVE);  // isor1 <= 127, isor2 <= 1000 rows
+  ctx->hres_threads = std::min<int64_t>(mp, (int64_t)prop.multiProcessorCount * 1024);
+  ctx->rsd_part_len = std::max<int64_t>(128 * (mp / WAVE), 1000 * (ctx->hres_threads / WAVE));
   HIPCHK(hipMalloc(&ctx->rsd_part, sizeof(double) * ctx->rsd_part_len));
   ctx->res_part_len = 2 * ((max_cells + 255) / 256 + 1);
   HIPCHK(hipMalloc(&ctx->res_part, sizeof(double) * ctx->res_part_len));
   HIPCHK(hipMalloc(&ctx->rows, sizeof(double) * 1024));
+  HIPCHK(hipMalloc(&ctx->counters, sizeof(unsigned long long) * 2));
+  if (UC_SWEEP_LDS * 512 > 65536)
+    HIPCHK(hipFuncSetAttribute((const void*)k_sweep, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               UC_SWEEP_LDS * 512));
   for (auto& e : ctx->ev) HIPCHK(hipEventCreate(&e));
   *out = ctx.release();
   return UC_OK;
@@ -571,9 +675,10 @@ int uc_create(const uc_scales* scales, int64_t max_cells, uc_context** out) {
 
 int uc_destroy(uc_context* ctx) {
   if (!ctx) return UC_OK;
-  for (void* p : {(void*)ctx->dp, (void*)ctx->rcs, (void*)ctx->sa, (void*)ctx->sst, (void*)ctx->rhsz,
-                  (void*)ctx->nsw, (void*)ctx->cells, (void*)ctx->redo, (void*)ctx->done,
-                  (void*)ctx->rsd_part, (void*)ctx->res_part, (void*)ctx->rows})
+  for (void* p : {(void*)ctx->dp, (void*)ctx->slabs, (void*)ctx->st, (void*)ctx->snap, (void*)ctx->sst2p,
+                  (void*)ctx->sa, (void*)ctx->sst, (void*)ctx->rhsz, (void*)ctx->nsw, (void*)ctx->cells,
+                  (void*)ctx->done, (void*)ctx->rsd_part, (void*)ctx->res_part, (void*)ctx->rows,
+                  (void*)ctx->counters})
     (void)hipFree(p);
   for (auto& e : ctx->ev) (void)hipEventDestroy(e);
   delete ctx;

@@ -28,6 +28,8 @@ The loss and production sums follow the reference's term order.
 from __future__ import annotations
 
 import json
+import os
+import re
 import sys
 from pathlib import Path
 
@@ -206,6 +208,206 @@ def reactants(t):
     return reac
 
 
+SLOT_BASE = {"UC_SLOT_RHOFA": 544, "UC_SLOT_G": 595, "UC_SLOT_SOUR": 647, "UC_SLOT_SO": 653}
+RC_READ = re.compile(r"rc\.ld\(([^()]*)\)")
+
+# register cache and prefetch depth of the sweep's slab reads (doubles)
+SWEEP_CACHE = int(os.environ.get("UC_SWEEP_CACHE", "100"))
+SWEEP_PREFETCH = int(os.environ.get("UC_SWEEP_PREFETCH", "32"))
+SWEEP_LDS = int(os.environ.get("UC_SWEEP_LDS", "126"))     # LDS positions (doubles per lane)
+
+
+def slot_value(expr: str) -> int:
+    expr = expr.strip()
+    if "+" in expr:
+        a, b = expr.split("+")
+        return SLOT_BASE[a.strip()] + int(b)
+    return SLOT_BASE[expr] if expr in SLOT_BASE else int(expr)
+
+
+def belady_plan(reads, K):
+    """For a fixed sequence of slab reads, decide which reads reuse a value held in
+    a register (an optimal, farthest-next-use cache of K doubles).  Returns, per
+    read, the index of the load that supplies it, and the list of loads (slots)."""
+    INF = 1 << 60
+    n = len(reads)
+    nxt = [INF] * n
+    last = {}
+    for j in range(n - 1, -1, -1):
+        nxt[j] = last.get(reads[j], INF)
+        last[reads[j]] = j
+    cache = {}          # slot -> (load index, next use)
+    src = [0] * n
+    loads = []
+    for j, slot in enumerate(reads):
+        if slot in cache:
+            li, _ = cache[slot]
+            src[j] = li
+            if nxt[j] == INF:
+                del cache[slot]
+            else:
+                cache[slot] = (li, nxt[j])
+            continue
+        li = len(loads)
+        loads.append(slot)
+        src[j] = li
+        if nxt[j] == INF or K == 0:
+            continue
+        if len(cache) < K:
+            cache[slot] = (li, nxt[j])
+            continue
+        far = max(cache, key=lambda s_: cache[s_][1])
+        if cache[far][1] > nxt[j]:
+            del cache[far]
+            cache[slot] = (li, nxt[j])
+    return src, loads
+
+
+def _plan(reads, K, KL, C, stmt_of=None):
+    """Serve a fixed sequence of slab reads from K registers and KL LDS positions.
+
+    One farthest-next-use cache of K + KL values decides which reads are hits.  A hit
+    read j reuses the value of the previous read p of its slot; the gap [p, j) is held
+    either in a register or in an LDS position.  Registers take a maximum set of gaps
+    with at most K overlapping (left-to-right sweep, dropping the gap that ends last);
+    the other gaps, at most KL overlapping, get LDS positions by interval colouring.
+    Returns per read: ("load",), ("reg", p) or ("lds", p, pos); and the LDS stores
+    per read index (the value of read p stored to pos after p's statement)."""
+    INF = 1 << 60
+    n = len(reads)
+    nxt = [INF] * n
+    last = {}
+    for j in range(n - 1, -1, -1):
+        nxt[j] = last.get(reads[j], INF)
+        last[reads[j]] = j
+    cache = {}               # slot -> (read index of the latest read, next use)
+    prev = [None] * n        # for hits: the previous read of the same value
+    for j, slot in enumerate(reads):
+        if slot in cache:
+            prev[j] = cache[slot][0]
+            if nxt[j] == INF:
+                del cache[slot]
+            else:
+                cache[slot] = (j, nxt[j])
+            continue
+        if nxt[j] == INF or C == 0:
+            continue
+        if len(cache) >= C:
+            far = max(cache, key=lambda s_: cache[s_][1])
+            if cache[far][1] <= nxt[j]:
+                continue
+            del cache[far]
+        cache[slot] = (j, nxt[j])
+    gaps = sorted((prev[j], j) for j in range(n) if prev[j] is not None)
+    import heapq
+    in_reg, active, ends, far = set(), set(), [], []
+    for a, b in gaps:
+        while ends and ends[0][0] <= a:          # gaps over by now
+            e, s_ = heapq.heappop(ends)
+            active.discard((s_, e))
+        in_reg.add((a, b))
+        if stmt_of is not None and stmt_of[a] == stmt_of[b]:
+            continue                              # within one statement: a live temporary
+        active.add((a, b))
+        heapq.heappush(ends, (b, a))
+        heapq.heappush(far, (-b, a))
+        if len(active) > K:                       # the gap ending last leaves the registers
+            while True:
+                nb, s_ = heapq.heappop(far)
+                if (s_, -nb) in active:
+                    break
+            active.discard((s_, -nb))
+            in_reg.discard((s_, -nb))
+    plan = [("load",)] * n
+    stores = {}
+    free, busy = list(range(KL - 1, -1, -1)), []   # busy: (end, pos)
+    for a, b in gaps:
+        if (a, b) in in_reg:
+            plan[b] = ("reg", a)
+            continue
+        while busy and busy[0][0] <= a:
+            free.append(heapq.heappop(busy)[1])
+        if not free:
+            return None
+        pos = free.pop()
+        heapq.heappush(busy, (b, pos))
+        plan[b] = ("lds", a, pos)
+        stores.setdefault(a, []).append(pos)
+    return plan, stores
+
+
+def plan_two_level(reads, K, KL, stmt_of=None):
+    """_plan with the largest cache capacity C >= K whose LDS demand fits KL."""
+    lo, hi = K, K + 4 * KL + 1
+    best = _plan(reads, K, KL, K, stmt_of)
+    while lo < hi:
+        mid = (lo + hi + 1) // 2
+        p = _plan(reads, K, KL, mid, stmt_of)
+        if p is None:
+            hi = mid - 1
+        else:
+            lo, best = mid, p
+    return best
+
+
+def emit_cached(stmts, L, K, M, KL=0, indent="  ", barrier="UC_SB();", transform=None):
+    """Emit straight-line statements whose slab reads rc.ld(slot) are served from
+    registers and the wave's LDS as plan_two_level decides.  Global loads are issued
+    up to M loads ahead of their first use; a value kept in LDS is stored after the
+    statement that read it and read back just before the statement that needs it.
+    A scheduling barrier follows each statement, so the order of loads and arithmetic
+    is the one written here.  transform(slot, var), if given, is the expression of the
+    value a read stands for (e.g. RC times species), formed once at the first use."""
+    reads, tmpl, stmt_of = [], [], []
+    for i, st in enumerate(stmts):
+        parts = RC_READ.split(st)   # text, slot, text, slot, ...
+        tmpl.append(parts)
+        for q in range(1, len(parts), 2):
+            reads.append(slot_value(parts[q]))
+            stmt_of.append(i)
+    plan, stores = plan_two_level(reads, K, KL, stmt_of)
+    var = [None] * len(reads)             # the variable serving each read
+    loads = [j for j in range(len(reads)) if plan[j][0] == "load"]
+    gid = {j: g for g, j in enumerate(loads)}
+    need = [-1] * len(stmts)
+    for j in loads:
+        need[stmt_of[j]] = max(need[stmt_of[j]], gid[j])
+    by_stmt = [[] for _ in stmts]
+    for j in range(len(reads)):
+        by_stmt[stmt_of[j]].append(j)
+    emitted, hi, nlds = 0, -1, 0
+    for i, parts in enumerate(tmpl):
+        hi = max(hi, need[i])
+        stop = min(len(loads), hi + 1 + M) if hi >= 0 else min(len(loads), M)
+        while emitted < stop:
+            j = loads[emitted]
+            L.append(f"{indent}const double g{j} = rc.ld({reads[j]});")
+            emitted += 1
+        for j in by_stmt[i]:
+            kind = plan[j][0]
+            if kind == "load":
+                var[j] = f"g{j}"
+                if transform is not None:
+                    L.append(f"{indent}const double a{j} = {transform(reads[j], var[j])};")
+                    var[j] = f"a{j}"
+            elif kind == "reg":
+                var[j] = var[plan[j][1]]
+            else:
+                L.append(f"{indent}const double w{j} = UC_LDS_LD({plan[j][2]});")
+                var[j] = f"w{j}"
+                nlds += 1
+        out = [parts[0]]
+        for q, j in zip(range(1, len(parts), 2), by_stmt[i]):
+            out.append(var[j])
+            out.append(parts[q + 1])
+        L.append(indent + "".join(out))
+        for j in by_stmt[i]:
+            for pos in stores.get(j, ()):
+                L.append(f"{indent}UC_LDS_ST({pos}, {var[j]});")
+        L.append(indent + barrier)
+    return len(reads), nlds, len(loads)
+
+
 def emit_sweep(t, L):
     reac = reactants(t)
     blocks = [b for b in t["sweep"]["blocks"] if b["kind"] != "soot"]
@@ -218,11 +420,18 @@ def emit_sweep(t, L):
                 doubled_species.add(j)
     L.append("// one Gauss-Seidel sweep of one cell (solvesp.f:87-2831), chemistry-only contract.")
     L.append("// s[0..51]: species iterates, s[52] soot mass fraction, s[53] soot number density.")
-    L.append("// rhofa[k] = rho*Y_k(old) for k < 51.  Returns sum |S - FNEW| over species 1..51 and")
-    L.append("// the soot mass fraction (RSD's per-cell share); *dsst2 receives |dSST2| (not in RSD).")
-    L.append("// rho*Y_k(old) is read from slab slot UC_SLOT_RHOFA+k, DT*W_k from P->dtw[k].")
+    L.append("// Returns sum |S - FNEW| over species 1..51 and the soot mass fraction (RSD's")
+    L.append("// per-cell share); *dsst2 receives |dSST2| (not in RSD).  Slab reads: RC, rho*Y_k(old)")
+    L.append("// (UC_SLOT_RHOFA+k), SOUR (UC_SLOT_SOUR+q); DT*W_k from P->dtw[k].  Each slab value is")
+    L.append(f"// loaded once and kept in a register while it is among the {SWEEP_CACHE} soonest reused,")
+    L.append(f"// else in the lane's {SWEEP_LDS} LDS positions (farthest-next-use replacement at both")
+    L.append(f"// levels, planned here); global loads run {SWEEP_PREFETCH} ahead of use.  lds: this lane's")
+    L.append("// LDS positions, stride 64 doubles.")
+    L.append(f"#define UC_SWEEP_LDS {SWEEP_LDS}")
+    L.append("#define UC_LDS_LD(pos) (lds[(pos) * 64])")
+    L.append("#define UC_LDS_ST(pos, v) (lds[(pos) * 64] = (v))")
     L.append("template <class RcPtr>")
-    L.append("__device__ __forceinline__ double uc_sweep(UcP P, RcPtr rc, double (&s)[54],")
+    L.append("__device__ __forceinline__ double uc_sweep(UcP P, RcPtr rc, double* __restrict__ lds, double (&s)[54],")
     L.append("    double rho, double stmf, double stnd, double dt, double sor1, double sor2, double* dsst2) {")
     L.append("  double rsd = 0.0, rhs1a = 0.0;")
     so_slot = {j: i for i, j in enumerate(sorted(doubled_species))}
@@ -230,10 +439,21 @@ def emit_sweep(t, L):
     SO_SLOT.update(so_slot)
     for j, i in so_slot.items():
         L.append(f"  rc.st(UC_SLOT_SO + {i}, s[{j - 1}]);")
+    L.append("  UC_SB();")
+    S = []   # flat statements, each followed by a barrier
+
+    def chunked(var, terms, init=True):
+        decl = "double " if init else ""
+        if not terms:
+            S.append(f"{decl}{var} = 0.0;")
+            return
+        for i in range(0, len(terms), CHUNK):
+            part = " + ".join(terms[i:i + CHUNK])
+            S.append(f"{decl if i == 0 else ''}{var} = {part};" if i == 0 else f"{var} = {var} + {part};")
+
     done = set()
     for b in blocks:
         k = b["k"]
-        L.append(f"  {{ // species {k}")
         if b["rhs1a"] is not None:
             terms = []
             for term in b["rhs1a"]:
@@ -243,13 +463,9 @@ def emit_sweep(t, L):
                     c = f"{f32(term['coef'])} * " if term.get("coef") else ""
                     x = f" * s[{term['times'] - 1}]" if term.get("times") else ""
                     terms.append(f"{c}rc.ld(UC_SLOT_SOUR + {term['sour'] - 1}){x}")
-            chunked_sum("rhs1a", terms, L, init=False)
-            L.append("    UC_SB();")
-        bt = []
-        for term in b["rhs1b"]:
-            bt.append(prod_term(term["slot"], reac, done, k, stale_found, doubled_species))
-        chunked_sum("rhs1b", bt, L)
-        L.append("    UC_SB();")
+            chunked("rhs1a", terms, init=False)
+        bt = [prod_term(term["slot"], reac, done, k, stale_found, doubled_species) for term in b["rhs1b"]]
+        chunked(f"rhs1b_{k}", bt)
         lt = []
         for term in b["rhs2"]:
             if "slot" in term:
@@ -260,36 +476,30 @@ def emit_sweep(t, L):
             else:
                 c = f"{f32(term['coef'])} * " if term.get("coef") else ""
                 lt.append(f"{c}rc.ld(UC_SLOT_SOUR + {term['sour'] - 1})")
-        chunked_sum("loss", lt, L)
-        L.append(f"    const double rhs1 = rc.ld(UC_SLOT_RHOFA + {k - 1}) + P->dtw[{k - 1}] * (rhs1a + rhs1b);")
-        L.append(f"    const double rhs2 = -P->dtw[{k - 1}] * loss;")
-        L.append(f"    double fnew = sor2 * s[{k - 1}] + rhs1 * sor1 / (rho - rhs2);")
-        L.append(f"    rsd += fabs(s[{k - 1}] - fnew);")
-        L.append(f"    s[{k - 1}] = fnew < 0.0 ? 0.0 : fnew;")
-        L.append("    asm volatile(\"\" : \"+v\"(rsd));  // keep the residual sum current (no sinking)")
-        L.append("  }")
-        L.append("  UC_SB();")
+        chunked(f"loss_{k}", lt)
+        S.append(f"const double rhs1_{k} = rc.ld(UC_SLOT_RHOFA + {k - 1}) + P->dtw[{k - 1}] * (rhs1a + rhs1b_{k});"
+                 f" const double rhs2_{k} = -P->dtw[{k - 1}] * loss_{k};"
+                 f" const double fnew_{k} = sor2 * s[{k - 1}] + rhs1_{k} * sor1 / (rho - rhs2_{k});"
+                 f" rsd += fabs(s[{k - 1}] - fnew_{k}); s[{k - 1}] = fnew_{k} < 0.0 ? 0.0 : fnew_{k};"
+                 " asm volatile(\"\" : \"+v\"(rsd));")
         done.add(k)
     missing = stale_expected - stale_found
     extra = stale_found - stale_expected
     assert not missing and not extra, (missing, extra)
     # soot (solvesp.f:2807-2831)
-    L.append("  { // soot mass fraction and number density")
-    L.append("    double sour[6];")
-    L.append("    for (int q = 0; q < 6; ++q) sour[q] = rc.ld(UC_SLOT_SOUR + q);")
-    L.append(f"    const double rhs1 = rho * stmf + dt * ({f32('0.012011')} / P->wmstr) * (2.0 * sour[0] * s[21]"
-             " - sour[2] * s[1] - sour[3] * s[4] - sour[4] * s[3]);")
-    L.append("    double fnew = sor2 * s[52] + rhs1 * sor1 / rho;")
-    L.append("    rsd += fabs(s[52] - fnew);")
-    L.append("    s[52] = fnew < 0.0 ? 0.0 : fnew;")
-    L.append("    const double rhs1n = rho * stnd + dt * sour[1] * s[21];")
-    L.append("    fnew = sor2 * s[53] + rhs1n * sor1 / (rho + dt * sour[5]);")
-    L.append("    fnew = fnew < 0.0 ? 0.0 : fnew;")
-    L.append("    *dsst2 = fabs(s[53] - fnew);")
-    L.append("    s[53] = fnew;")
-    L.append("  }")
+    sr = [f"rc.ld(UC_SLOT_SOUR + {q})" for q in range(6)]
+    S.append(f"const double rhs1_soot = rho * stmf + dt * ({f32('0.012011')} / P->wmstr) * (2.0 * {sr[0]} * s[21]"
+             f" - {sr[2]} * s[1] - {sr[3]} * s[4] - {sr[4]} * s[3]);"
+             " const double fnew_sm = sor2 * s[52] + rhs1_soot * sor1 / rho;"
+             " rsd += fabs(s[52] - fnew_sm); s[52] = fnew_sm < 0.0 ? 0.0 : fnew_sm;")
+    S.append(f"const double rhs1n = rho * stnd + dt * {sr[1]} * s[21];"
+             f" double fnew_sn = sor2 * s[53] + rhs1n * sor1 / (rho + dt * {sr[5]});"
+             " fnew_sn = fnew_sn < 0.0 ? 0.0 : fnew_sn; *dsst2 = fabs(s[53] - fnew_sn); s[53] = fnew_sn;")
+    nreads, nlds, nloads = emit_cached(S, L, SWEEP_CACHE, SWEEP_PREFETCH, SWEEP_LDS)
+    L.append("  (void)lds;")
     L.append("  return rsd;")
     L.append("}")
+    L.append(f"// slab reads per sweep: {nreads} written, {nlds} from LDS, {nloads} loaded")
     L.append("")
     return stale_found
 
@@ -321,13 +531,20 @@ def prod_term(slot, reac, done, k, stale_found, doubled_species):
     return "rc.ld(%d)%s" % (slot - 1, "".join(" * " + f for f in fac))
 
 
+PRD_CACHE = int(os.environ.get("UC_PRD_CACHE", "64"))
+PRD_PREFETCH = int(os.environ.get("UC_PRD_PREFETCH", "24"))
+
+
 def emit_prd(t, L):
     reac = reactants(t)
     L.append("// net production PRD_k (solveh.f:121-1131) with the final iterates s[0..51]:")
     L.append("// pr[k] = sum over reactions of +-RA (RA = RC*prod S); PRD_k = WM_k HFO_k pr[k].")
+    L.append(f"// Each RA is formed once and kept in a register while among the {PRD_CACHE} soonest")
+    L.append("// reused (the reference also forms RA(r) once, solveh.f:117-669, and sums it).")
     L.append("template <class RcPtr, class Emit>")
     L.append("__device__ __forceinline__ void uc_prd(UcP P, RcPtr rc, const double (&s)[54], Emit emit) {")
     prd = t["prd"]
+    S = []
     for k in range(1, NSP + 1):
         terms = prd.get(str(k), [])
         parts = []
@@ -337,22 +554,20 @@ def emit_prd(t, L):
             sl = term["idx"]
             if sl not in reac:
                 continue          # never-assigned RA slots read as zero
-            f = "".join(f" * s[{j - 1}]" for j in reac[sl])
-            parts.append(f"{term['sign']} rc.ld({sl - 1}){f}")
+            parts.append(f"{term['sign']} rc.ld({sl - 1})")
         if k >= 50 or not parts:
-            L.append(f"  emit(P, {k - 1}, 0.0);")
-        else:
-            L.append("  {")
-            terms = [parts[0][2:] if parts[0].startswith("+ ") else "-" + parts[0][2:]]
-            terms += [p_ for p_ in parts[1:]]
-            # keep the signs: build "acc = acc + x" / "acc = acc - x" in chunks
-            L.append("    double acc = 0.0;")
-            for i in range(0, len(parts), CHUNK):
-                L.append("    acc = acc " + " ".join(parts[i:i + CHUNK]) + ";")
-                L.append("    UC_SB();")
-            L.append(f"    emit(P, {k - 1}, acc);")
-            L.append("  }")
+            S.append(f"emit(P, {k - 1}, 0.0);")
+            continue
+        # "acc = acc +- x" in chunks, starting from 0.0 as the reference's PRD(k) = ... does
+        for i in range(0, len(parts), CHUNK):
+            head = "double " if i == 0 else ""
+            src = "0.0" if i == 0 else f"acc{k}"
+            S.append(f"{head}acc{k} = {src} " + " ".join(parts[i:i + CHUNK]) + ";")
+        S.append(f"emit(P, {k - 1}, acc{k});")
+    fac = {sl - 1: "".join(f" * s[{j - 1}]" for j in rs) for sl, rs in reac.items()}
+    n, nl, ng = emit_cached(S, L, PRD_CACHE, PRD_PREFETCH, 0, transform=lambda slot, v: v + fac[slot])
     L.append("}")
+    L.append(f"// RA reads: {n} written, {ng} loaded")
     L.append("")
 
 
