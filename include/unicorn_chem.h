@@ -13,7 +13,7 @@
  *      solveh_   replaces SUBROUTINE SOLVEH(ISOR2,RELXH,TOLRH,SIGSP,HNORM)
  *                                                                 (src/lib/solveh.f:1)
  *
- *    They live in libunicorn_fshim.so (unicorn_amd/csrc/fortran_shim.cpp),
+ *    They live in libunicorn_fshim.so (unicorn_amd/csrc/fortran_shim.hip),
  *    built for the grid PARAMETER(LI,LJ) of the Fortran build, and read and
  *    write the COMMON blocks /CB01/ /CB02/ /CB03/ /CB04/ /CB06/ /CB08/ /REAC/
  *    /HFORM/ /HRLS/ /DUMD/ /SOOT/ /DUMY/ /DUMS/ /DUMR/ /DUMMY/ /DUM0/ /DUMX/

@@ -33,34 +33,7 @@
 
 #include "uc_consts.h"
 
-#define LI 711
-#define LJ 131
-#define LE (LI * LJ)
-#define LSP 52
-#define LRX 544
-
-/* COMMON blocks of the reference (declared as in src/lib/*.f; symbol sizes
- * verified below against the values nm reports for the flang objects). */
-extern struct { int32_t isym, iflow, iswrl, ithrm, ichem, igrav, itr; int32_t iskip[LE]; } cb01_;
-extern struct { double rref, vref, tref, pref, wm[LSP]; } cb02_;
-extern struct { double dt, x[LE], y[LE], xxc[LI], xxs[LI], yyc[LJ], yys[LJ]; } cb03_;
-extern struct { double rho[LE], fsp[LSP][LE], u[LE], v[LE], w[LE], p[LE], ht[LE], tk[LE], ak[LE], eps[LE]; } cb04_;
-extern struct { double alstr, vstr, rstr, pstr, tstr, tmstr, amstr, acstr, dstr, hstr, cpstr, akstr,
-                epstr, wmstr, gasc, beta1, beta2, beta3, beta4, rei; } cb06_;
-extern struct { double vmu[LE], vtc[LE], vdf[LSP][LE], tmu[LE]; } cb07_;
-extern struct { double tpol1, tpol2, polsp[LSP][14], cisp[LSP][12]; } cb08_;
-extern struct { int32_t lrev, nrlind, itbef[LRX]; double alf[LRX], akf[LRX], exaf[LRX], allow[LRX],
-                aklow[LRX], ealow[LRX], troe[4][LRX]; } reac_;
-extern struct { double hfo[LSP]; } hform_;
-extern struct { double qdot[LE]; } hrls_;
-extern struct { double qdsp[LSP][LE]; } dumd_;
-extern struct { double rsoot, stmf[LE], stnd[LE], stdf[LE]; } soot_;
-extern struct { double a[55][LE]; } dummy_;
-extern struct { double a[149][LE]; } dum0_;
-extern struct { double ze[LE], zw[LE], zn[LE], zs[LE]; } dumx_;
-extern struct { double ssp[LSP][LE], fpz[LE], emu[LE], ekt[LE]; } dumy_;
-extern struct { double rc[LRX][LE]; } dumr_;
-extern struct { double ste[LE], stw[LE], stn[LE], sts[LE], sst1[LE], sst2[LE], sour[6][LE]; } dums_;
+#include "ref_commons.h"
 
 extern void ckrates_(void);
 extern void solvesp_(int32_t *isor1, double *relxsp, double *tolrsp, double *spnorm);

@@ -16,12 +16,15 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 DRIVER = HERE / "_ref" / "ref_driver"
+# the same driver linked against the gfx950 drop-in (libunicorn_fshim.so) in place of
+# the flang-built CKRATES/SOLVESP/SOLVEH: runs the shipped Fortran ABI on the GPU
+SHIM_DRIVER = HERE / "_ref" / "ref_driver_shim"
 LSP, LRX = 52, 544
 FLAG_FORCE_SWEEPS = 1
 
 
-def available() -> bool:
-    return DRIVER.exists()
+def available(driver: str = "ref") -> bool:
+    return (SHIM_DRIVER if driver == "shim" else DRIVER).exists()
 
 
 @dataclass
@@ -98,11 +101,14 @@ def read_output(path, field, nsteps, ndump) -> RefResult:
                      qdot, qdsp, t_chem)
 
 
-def run(field, nsteps=1, dump_cells=(), **kw) -> RefResult:
-    if not available():
-        raise FileNotFoundError(f"{DRIVER} not built (make -C oracle ref)")
+def run(field, nsteps=1, dump_cells=(), driver="ref", **kw) -> RefResult:
+    """Run the driver ("ref": flang-built reference routines; "shim": the gfx950
+    drop-in) on a field; raises CalledProcessError (with its stderr) on failure."""
+    exe = SHIM_DRIVER if driver == "shim" else DRIVER
+    if not exe.exists():
+        raise FileNotFoundError(f"{exe} not built (make -C oracle ref shim)")
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         write_input(fin, field, nsteps=nsteps, dump_cells=dump_cells, **kw)
-        subprocess.run([str(DRIVER), "run", fin, fout], check=True)
+        subprocess.run([str(exe), "run", fin, fout], check=True, capture_output=True, text=True)
         return read_output(fout, field, nsteps, len(dump_cells))

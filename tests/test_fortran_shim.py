@@ -1,0 +1,76 @@
+"""The Fortran drop-in: libunicorn_fshim.so exports CKRATES/SOLVESP/SOLVEH
+(ckrates_, solvesp_, solveh_) over the reference's COMMON blocks.
+
+oracle/_ref/ref_driver_shim is the reference driver (oracle/ref_driver.c) linked
+against that library instead of the flang-built routines, so these tests run
+the shipped Fortran ABI end to end: COMMON layout, argument lists, ISOR1/ISOR2
+in/out, the SPGSOLV epilogue and density update left to the caller.  On the GPU
+its outputs must match the reference's own (the golden fixtures) to the same
+bar as the device API.
+"""
+from __future__ import annotations
+
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from golden_io import CASES, assert_rates_close, assert_state_close, expected, load
+from oracle import refrun
+from unicorn_amd.fields import ChemField
+
+ROOT = Path(__file__).resolve().parents[1]
+SHIM_LIB = ROOT / "unicorn_amd" / "lib" / "libunicorn_fshim.so"
+COMMONS = ("cb01_", "cb02_", "cb03_", "cb04_", "cb06_", "hform_", "hrls_", "dumd_", "soot_", "dummy_",
+           "dum0_", "dumx_", "dumy_", "dums_")
+
+
+def _nm(args):
+    return subprocess.run(["nm", "-D", *args, str(SHIM_LIB)], check=True, capture_output=True,
+                          text=True).stdout
+
+
+def test_shim_exports_fortran_names_and_binds_commons():
+    if not SHIM_LIB.exists():
+        pytest.skip("libunicorn_fshim.so not built")
+    defined = {l.split()[-1] for l in _nm(["--defined-only"]).splitlines() if l.strip()}
+    undefined = {l.split()[-1] for l in _nm(["--undefined-only"]).splitlines() if l.strip()}
+    assert {"ckrates_", "solvesp_", "solveh_"} <= defined
+    # the COMMON blocks belong to the Fortran program, not to the library
+    assert set(COMMONS) <= undefined
+    assert not (set(COMMONS) & defined)
+
+
+def test_shim_without_gpu_stops_with_a_message():
+    import torch
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is present")
+    if not refrun.available("shim"):
+        pytest.skip("ref_driver_shim not built")
+    f0, _ = load("sooty")
+    with pytest.raises(subprocess.CalledProcessError) as e:
+        refrun.run(f0, nsteps=1, driver="shim")
+    assert e.value.returncode == 3
+    assert "unicorn_fshim: ckrates_:" in e.value.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_shim_matches_reference_fixtures(name):
+    if not refrun.available("shim"):
+        pytest.skip("ref_driver_shim not built")
+    f0, z = load(name)
+    inter = f0.interior()
+    for ns in [int(s) for s in z["steps"]]:
+        print(f"shim case {name} steps {ns}", flush=True)
+        r = refrun.run(f0.copy(), nsteps=ns, driver="shim")
+        e = expected(z, ns)
+        assert list(r.isor1) == list(e["isor1"])
+        assert list(r.isor2) == list(e["isor2"])
+        assert np.allclose(r.resdc, e["resdc"], rtol=1e-9, atol=0)
+        assert np.allclose(r.resdh, e["resdh"], rtol=1e-9, atol=0)
+        got = ChemField(f0.li, f0.lj, r.rho, r.y, r.tk, r.ht, r.stmf, r.stnd, f0.iskip, f0.dt, f0.fuel)
+        assert_state_close(got, e, inter, f"shim {name}/{ns}")
+        dump = z["dump_cells"]
+        assert_rates_close(r.qdot, r.qdsp[:, dump], e["qdot"], e["qdsp_dump"], inter, f"shim {name}/{ns}")

@@ -39,6 +39,17 @@ def build_chem(verbose=False) -> Path:
     return out
 
 
+def build_fshim(li: int = 711, lj: int = 131) -> Path:
+    """The Fortran drop-in (ckrates_/solvesp_/solveh_ over the COMMON blocks) for a
+    Fortran build with PARAMETER (LI=li, LJ=lj); its COMMON symbols bind to the program."""
+    LIBDIR.mkdir(exist_ok=True)
+    out = LIBDIR / "libunicorn_fshim.so"
+    tmp = LIBDIR / "libunicorn_fshim.so.tmp"
+    run([HIPCC, *HIP_FLAGS, f"-DUC_LI={li}", f"-DUC_LJ={lj}", CSRC / "fortran_shim.hip", "-o", tmp], cwd=CSRC)
+    tmp.replace(out)
+    return out
+
+
 def main():
     build_chem()
 

@@ -444,6 +444,7 @@ struct FinishArgs {
   int32_t mode;
   double zsor1, zsor2, tolrh;
   double* res_part;  // [2][nblocks] RESDC, RESDH partial sums
+  double* fpz_out;   // null: SPGSOLV epilogue + density; else FPZ here and TK only
 };
 
 __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg, FinishArgs a) {
@@ -505,6 +506,12 @@ __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg,
     if (tkd > 5000.0) tkd = 5000.0;
     if (tkd < P->tstr - 1.0) tkd = P->tstr - 1.0;
     const double tnew = tkd / P->tstr;
+    if (a.fpz_out) {  // SOLVEH's outputs only (solveh.f:1167, :1230)
+      a.fpz_out[c] = fpz;
+      a.tk[c] = tnew;
+      goto reduce;
+    }
+    {
     // SPGSOLV epilogue (spgsolv.f:1299-1416)
     const double sst1 = a.sst[q], sst2 = a.sst[a.npad + q];
     rh_ = fabs(fpz - ht);
@@ -521,7 +528,9 @@ __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg,
     a.stnd[c] = sst2;
     a.tk[c] = tnew;
     a.rho[c] = P->pback / (P->beta1 * tnew * rbar);  // main:1056-1061
+    }
   }
+reduce:
   __shared__ double red[2][256 / WAVE];
   rc_ = wave_sum(rc_);
   rh_ = wave_sum(rh_);
@@ -568,6 +577,22 @@ __global__ void k_zero(double* __restrict__ p, int64_t n) {
 }  // namespace
 
 // ---------------------------------------------------------------- host side
+namespace {
+// bookkeeping of the step in progress (chem_host.inc: ph_begin .. ph_end)
+struct StepRun {
+  const uc_field* f = nullptr;
+  uc_controls ctl{};
+  hipStream_t s = nullptr;
+  int64_t n = 0, npad = 0, nwc = 0, nb = 0;
+  int32_t ncells = 0, nstar = 0, nh = 0, snap_from = 1;
+  bool global = true;
+  double sor1 = 0.0, sor2 = 0.0, spnorm = 0.0, hnorm = 0.0;
+  float ms_rates = 0.f, ms_sweeps = 0.f, ms_sources = 0.f;
+  uc_stats out{};
+  std::vector<double> r = std::vector<double>(1024, 0.0);  // per-sweep residual sums
+};
+}  // namespace
+
 struct uc_context {
   uc_scales sc{};
   UcParams hp{};
@@ -600,6 +625,7 @@ struct uc_context {
   const int32_t* mask_ptr = nullptr;
   int32_t mask_li = 0, mask_lj = 0;
   std::vector<int32_t> cell_list;
+  StepRun run;
 };
 
 namespace {
