@@ -102,6 +102,15 @@ typedef struct {
     double *qdsp;           /* [52][n] out: species production (solveh.f:1137) */
 } uc_field;
 
+/* Tuning of a step (uc_autotune chooses it; uc_set_tuning / uc_get_tuning). */
+typedef struct {
+    int32_t bins;       /* 1: cells in natural order; 2..32: each grid's cells ordered
+                           stiffest first by the previous step's sweeps in this many bins */
+    int32_t lpt;        /* 1: tiles of 64 cells run stiffest first (longest-processing-time) */
+    int32_t persistent; /* 1: the sweep kernel runs persistent waves over the tile queue */
+    int32_t block;      /* block size of the one-thread-per-cell kernels: 64, 128 or 256 */
+} uc_tuning;
+
 typedef struct uc_context uc_context;
 
 int uc_abi_version(void);
@@ -117,19 +126,39 @@ int uc_scales_from_card(double vref, double tref, double pref, double rtin, doub
  * saved iterates and snapshots, sources; about 9.7 KB per cell) and the
  * sweep-residual partial sums; it allocates with hipMalloc. */
 int uc_create(const uc_scales *scales, int64_t max_cells, uc_context **out);
+
+/* A context for batches of up to max_grids grids of max_cells cells in total
+ * (max_cells + 64*max_grids <= 4,000,000).  uc_create is the one-grid case. */
+int uc_create_batch(const uc_scales *scales, int64_t max_cells, int32_t max_grids, uc_context **out);
 int uc_destroy(uc_context *ctx);
 
 /* One chemistry step on a device-resident field, on `stream` (a hipStream_t,
  * NULL = default stream).  Sequence (the reference around SPGSOLV):
  * main:821-828 clip/closure, CKRATES, SOLVESP, SOLVEH, the SPGSOLV epilogue,
  * and the ideal-gas density update main:1056-1073.  Synchronises the stream
- * (the global stops are decided on the host). */
+ * (the global stops are decided on the host).  The list of active cells is
+ * cached per (ISKIP pointer, LI, LJ): a caller that changes ISKIP in place
+ * passes a new array (or a new context). */
 int uc_step(uc_context *ctx, const uc_field *field, const uc_controls *ctl, void *stream,
             uc_stats *stats);
 
 /* Same, on host arrays (copies in and out; the PCIe cost is in ms_total). */
 int uc_step_host(uc_context *ctx, const uc_field *host_field, const uc_controls *ctl,
                  uc_stats *stats);
+
+/* One chemistry step of ngrids independent grids (each its own SPGSOLV call: its own
+ * DT, controls and global stops) in one set of launches; stats[ngrids].  The grids'
+ * cells are packed into the context, binned by stiffness and run as one work queue.
+ * No counterpart in the reference (its DT is a single COMMON scalar, /CB03/). */
+int uc_step_batch(uc_context *ctx, const uc_field *fields, const uc_controls *ctls,
+                  int32_t ngrids, void *stream, uc_stats *stats);
+
+/* Choose the tuning for these grids by timing steps on a copy of their state (the
+ * fields are left as they were); the choice stays in the context. */
+int uc_autotune(uc_context *ctx, const uc_field *fields, const uc_controls *ctls,
+                int32_t ngrids, void *stream, uc_tuning *chosen);
+int uc_set_tuning(uc_context *ctx, const uc_tuning *tuning);
+int uc_get_tuning(const uc_context *ctx, uc_tuning *tuning);
 
 /* ---- Fortran drop-in (libunicorn_fshim.so) ------------------------------ */
 void ckrates_(void);

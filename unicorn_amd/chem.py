@@ -45,13 +45,18 @@ class UcStats(C.Structure):
                 ("ms_total", C.c_double), ("sweep_launches", C.c_int32), ("first_segment", C.c_int32)]
 
 
+class UcTuning(C.Structure):
+    _fields_ = [("bins", C.c_int32), ("lpt", C.c_int32), ("persistent", C.c_int32), ("block", C.c_int32)]
+
+
 class UcField(C.Structure):
     _fields_ = [("li", C.c_int32), ("lj", C.c_int32)] + [
         (n, C.c_void_p) for n in ("rho", "y", "tk", "ht", "stmf", "stnd", "iskip", "qdot", "qdsp")]
 
 
-EXPORTS = ("uc_abi_version", "uc_last_error", "uc_scales_from_card", "uc_create", "uc_destroy",
-           "uc_step", "uc_step_host")
+EXPORTS = ("uc_abi_version", "uc_last_error", "uc_scales_from_card", "uc_create", "uc_create_batch",
+           "uc_destroy", "uc_step", "uc_step_host", "uc_step_batch", "uc_autotune", "uc_set_tuning",
+           "uc_get_tuning")
 
 _lib = None
 
@@ -73,10 +78,17 @@ def library() -> C.CDLL:
         raise ChemError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
     lib = C.CDLL(str(LIB_PATH))
     lib.uc_last_error.restype = C.c_char_p
-    for name in ("uc_scales_from_card", "uc_create", "uc_destroy", "uc_step", "uc_step_host"):
+    for name in EXPORTS[2:]:
         getattr(lib, name).restype = C.c_int
     lib.uc_scales_from_card.argtypes = [C.c_double] * 6 + [C.c_int32, C.POINTER(UcScales)]
     lib.uc_create.argtypes = [C.POINTER(UcScales), C.c_int64, C.POINTER(C.c_void_p)]
+    lib.uc_create_batch.argtypes = [C.POINTER(UcScales), C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]
+    lib.uc_step_batch.argtypes = [C.c_void_p, C.POINTER(UcField), C.POINTER(UcControls), C.c_int32, C.c_void_p,
+                                  C.POINTER(UcStats)]
+    lib.uc_autotune.argtypes = [C.c_void_p, C.POINTER(UcField), C.POINTER(UcControls), C.c_int32, C.c_void_p,
+                                C.POINTER(UcTuning)]
+    lib.uc_set_tuning.argtypes = [C.c_void_p, C.POINTER(UcTuning)]
+    lib.uc_get_tuning.argtypes = [C.c_void_p, C.POINTER(UcTuning)]
     lib.uc_destroy.argtypes = [C.c_void_p]
     lib.uc_step.argtypes = [C.c_void_p, C.POINTER(UcField), C.POINTER(UcControls), C.c_void_p,
                             C.POINTER(UcStats)]
@@ -162,11 +174,11 @@ class DeviceField:
 class ChemStep:
     """A chemistry-step context on the current GPU (scales + scratch)."""
 
-    def __init__(self, max_cells: int, scales: UcScales | None = None):
+    def __init__(self, max_cells: int, scales: UcScales | None = None, max_grids: int = 1):
         self.lib = library()
         self.scales = scales or scales_from_card()
         h = C.c_void_p()
-        _check(self.lib.uc_create(C.byref(self.scales), int(max_cells), C.byref(h)))
+        _check(self.lib.uc_create_batch(C.byref(self.scales), int(max_cells), int(max_grids), C.byref(h)))
         self.h = h
 
     def close(self):
@@ -190,6 +202,39 @@ class ChemStep:
         cc = ctl.c()
         _check(self.lib.uc_step(self.h, C.byref(fc), C.byref(cc), C.c_void_p(stream), C.byref(st)))
         return StepStats.of(st)
+
+    def step_batch(self, fields: list[DeviceField], ctls: list[Controls], stream=None) -> list[StepStats]:
+        """Advance several device-resident grids by one chemistry step each, in one batch."""
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        n = len(fields)
+        fa = (UcField * n)(*[f.c() for f in fields])
+        ca = (UcControls * n)(*[c.c() for c in ctls])
+        sa = (UcStats * n)()
+        _check(self.lib.uc_step_batch(self.h, fa, ca, n, C.c_void_p(stream), sa))
+        return [StepStats.of(x) for x in sa]
+
+    def autotune(self, fields: list[DeviceField], ctls: list[Controls], stream=None) -> dict:
+        """Pick the tuning for these grids (their state is left unchanged)."""
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        n = len(fields)
+        fa = (UcField * n)(*[f.c() for f in fields])
+        ca = (UcControls * n)(*[c.c() for c in ctls])
+        t = UcTuning()
+        _check(self.lib.uc_autotune(self.h, fa, ca, n, C.c_void_p(stream), C.byref(t)))
+        return {n_: getattr(t, n_) for n_, _ in UcTuning._fields_}
+
+    def set_tuning(self, bins=1, lpt=0, persistent=1, block=256):
+        t = UcTuning(bins, lpt, persistent, block)
+        _check(self.lib.uc_set_tuning(self.h, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = UcTuning()
+        _check(self.lib.uc_get_tuning(self.h, C.byref(t)))
+        return {n_: getattr(t, n_) for n_, _ in UcTuning._fields_}
 
     def step_host(self, f: ChemField, ctl: Controls) -> tuple[StepStats, np.ndarray, np.ndarray]:
         """Advance a host field (copied to and from the GPU); returns stats, QDOT, QDSP."""

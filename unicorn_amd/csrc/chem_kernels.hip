@@ -1,27 +1,25 @@
 // MI355X (gfx950) kernels and host C ABI of UNICORN's chemistry step.
 //
 // Step = main:821-828 clip/closure -> CKRATES -> SOLVESP -> SOLVEH -> SPGSOLV
-// epilogue -> main:1056-1073 density, for the chemistry-only contract.
+// epilogue -> main:1056-1073 density, for the chemistry-only contract, on one
+// grid or a batch of grids (each with its own DT, controls and global stops).
 //
-// Kernels (one thread per cell, FP64 throughout):
+// Kernels (FP64 throughout; one thread per position of the cell list, whose
+// grids are padded to whole waves):
 //   k_clip      all cells: Y<0 -> 0, Y_N2 = 1 - sum(Y_1..51)            main:821-828
-//   k_chem      fused, per active interior cell: rate coefficients into the
-//               thread's RC slab, Gauss-Seidel sweeps with the slab re-read
-//               each sweep, N2 closure, SOLVEH sources (radiation, RA/PRD,
-//               QDOT, QDSP, RHSZ).  A cell stops sweeping once a sweep leaves
-//               every iterate bit-for-bit unchanged (later sweeps would repeat
-//               it exactly); per-sweep residual sums are accumulated per wave
-//               so the host can find the reference's global stop N*.
-//   k_hres      per cell: enthalpy relaxation residual trajectory            solveh.f:1147-1167
-//   k_finish    per cell: enthalpy relaxation with the decided sweep count,
-//               Newton T(h), soot number density completion, SPGSOLV
-//               copy-back and residuals, density                             solveh.f:1170-1230
-//   k_density / k_density_edge  density of the remaining cells + extrapolation main:1056-1068
+//   k_rates     rate coefficients, rho*Y_k and soot sources into the cell's
+//               slab                                                      ckrates.f:92-1352
+//   k_sweep     a segment of Gauss-Seidel sweeps per tile of 64 cells, with
+//               per-tile per-sweep residual sums; a cell stops once a sweep
+//               leaves every iterate bit-for-bit unchanged               solvesp.f:87-2838
+//   k_sources   iterates at the decided count, N2 closure, SOLVEH sources  solvesp.f:2839-2914,
+//                                                                         solveh.f:34-1145
+//   k_hres      enthalpy relaxation residual trajectories                  solveh.f:1147-1167
+//   k_finish    enthalpy relaxation, Newton T(h), SPGSOLV epilogue, density solveh.f:1170-1234
+//   k_density / k_density_edge  density of the remaining cells, extrapolation main:1056-1068
+//   k_pack, k_bin_*, k_tile_keys, k_save_nsw   batches and the stiffness-sorted work queue
 //
-// RC slabs: thread t of a k_chem launch owns rc[s*G + t], s = 0..543 (G
-// threads in the launch), so every RC load/store of a wave is one contiguous
-// 512-byte segment.  A slab is written by the rate stage and read back by
-// every sweep of the same thread, so it stays in the L2 / Infinity Cache.
+// The step is orchestrated by chem_host.inc (phases ph_begin .. ph_end).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +31,8 @@
 #include <mutex>
 #include <string>
 #include <vector>
+
+#include <hipcub/device/device_radix_sort.hpp>
 
 #include "../../include/unicorn_chem.h"
 
@@ -48,7 +48,6 @@ struct UcParams {
   double wm[NSP], rwm[NSP], hfo[NSP];
   double nasa[NSP][14];
   double rcfac[NRX];  // C_m / prod W of the slot's reactants (ckrates.f:83-85, :796-1333)
-  double dtw[NSP];    // DT * WM(k) of the current step
 };
 
 typedef const __attribute__((address_space(4))) UcParams* UcP;
@@ -80,10 +79,9 @@ __device__ __forceinline__ uint64_t wave_uniform(uint64_t v) {
 
 constexpr int SLAB_SLOTS = 664;  // = UC_SLAB_SLOTS of chem_gen.inc
 
-__device__ __forceinline__ RcSlab wave_slab(double* slabs) {
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-  double* tile = (double*)wave_uniform((uint64_t)(slabs + wave * (uint64_t)SLAB_SLOTS * 64));
-  return RcSlab{__builtin_amdgcn_make_buffer_rsrc(tile, 0, SLAB_SLOTS * 512, 0x00020000),
+__device__ __forceinline__ RcSlab tile_slab(double* slabs, int64_t tile) {
+  double* t = (double*)wave_uniform((uint64_t)(slabs + tile * (int64_t)SLAB_SLOTS * 64));
+  return RcSlab{__builtin_amdgcn_make_buffer_rsrc(t, 0, SLAB_SLOTS * 512, 0x00020000),
                 (threadIdx.x & 63u) * 8u};
 }
 
@@ -149,14 +147,38 @@ __global__ void k_clip(double* __restrict__ y, int64_t n) {
 constexpr int32_t NSW_FIXED = 1 << 30;
 constexpr int32_t NSW_MASK = NSW_FIXED - 1;
 
-// Per-cell buffers are indexed by the cell's position q in the cell list;
-// species-major arrays have a row stride of npad (the list length rounded up
-// to whole waves).  The slab of cell q is tile q/64, lane q%64 -- thread q of a
-// one-thread-per-cell launch, so the tile is the launch's wave index.
+// Per-grid parameters and stop state of a (batched) step.  Grids are packed:
+// grid g's cells are field cells off .. off+li*lj-1 of the packed arrays.
+struct GridTab {
+  double dt, sor1, sor2, tolrsp, zsor1, zsor2, tolrh;
+  int32_t mode, isor1, isor2;
+  int32_t start, count, snap_from, redo;  // current sweep segment (counts are 1-based)
+  int32_t nstar, nh;                      // decided species / enthalpy sweep counts
+  int32_t hfrom;                          // first enthalpy sweep of the current k_hres chunk
+  int32_t tile0, tile1;                   // the grid's tiles of 64 list positions
+  int32_t li, lj;
+  int64_t off;
+};
+
+// The cell list: position q holds a packed field index, or -1-c for a padding lane
+// (c: a real cell of the same grid, so the lane has valid inputs).  Every grid's
+// positions are padded to whole tiles of 64, so a wave belongs to one grid.
+// Per-position buffers (slabs, st, snap, sa, ...) are indexed by q; species-major
+// ones have a row stride of npad (the list length).  The slab of position q is
+// tile q/64, lane q%64.
+__device__ __forceinline__ int64_t list_cell(const int32_t* cells, int64_t q, bool& valid) {
+  const int32_t v = cells[q];
+  valid = v >= 0;
+  return valid ? v : -1 - (int64_t)v;
+}
+
+__device__ __forceinline__ int32_t wave_grid(const int32_t* gid, int64_t q) {
+  return __builtin_amdgcn_readfirstlane(gid[q]);
+}
 
 struct RatesArgs {
   const double* rho; const double* y; const double* tk; const double* stmf; const double* stnd;
-  const int32_t* cells; int32_t ncells; int64_t n;
+  const int32_t* cells; int64_t npad; int64_t n;
   double* slabs;
 };
 
@@ -165,8 +187,10 @@ struct RatesArgs {
 __global__ void __launch_bounds__(256) k_rates(const UcParams* __restrict__ Pg, RatesArgs a) {
   UcP P = (UcP)Pg;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t c = a.cells[q < a.ncells ? q : a.ncells - 1];  // tail lanes redo the last cell
-  RcSlab rc = wave_slab(a.slabs);
+  if (q >= a.npad) return;  // whole waves only: npad is a multiple of 64
+  bool valid;
+  const int64_t c = list_cell(a.cells, q, valid);
+  RcSlab rc = tile_slab(a.slabs, q / WAVE);
   const FieldBuf Y = field_buf(a.y, a.n * 8 * NSP, c);
   const double rho = a.rho[c], tk = a.tk[c];
   double tbody;
@@ -181,60 +205,63 @@ __global__ void __launch_bounds__(256) k_rates(const UcParams* __restrict__ Pg, 
 
 struct SweepArgs {
   const double* rho; const double* y; const double* stmf; const double* stnd;
-  const int32_t* cells; int32_t ncells; int64_t n; int64_t npad;
+  const int32_t* cells; const int32_t* gid; const GridTab* gt;
+  int64_t npad; int64_t n;
   double* slabs;
   double* st;          // [54][npad] iterates where each cell stopped
   double* snap;        // [nsnap][54][npad] iterates after sweeps snap_from.. (may be null)
   double* sst2p;       // [npad] SST2 before a fixed cell's last sweep
   int32_t* nsw;        // [npad] NSW word
-  double* rsd_part;    // [isor1][rsd_stride] per-wave residual sums of each sweep (may be null)
-  int64_t rsd_stride;
-  unsigned long long* swcount;  // sweeps executed (+=)
-  int32_t* redo_count;          // cells re-run (+=, redo launches)
-  int32_t start, count;  // sweeps start+1 .. start+count (counts are 1-based)
-  int32_t snap_from;     // first sweep count that is snapshotted
+  double* rsd_part;    // [isor1][ntiles] per-tile residual sums of each sweep (may be null)
+  unsigned long long* swcount;  // [ngrids] sweeps executed (+=)
+  int32_t* redo_count;          // [ngrids] cells re-run (+=, redo launches)
+  const int32_t* order;  // tile order (null: natural)
+  int32_t* queue;        // persistent launches: next position in `order` (null: one tile per block)
+  int32_t ntiles;
   int32_t nsnap;
-  int32_t redo;          // > 0: re-run from scratch, to this count, the cells whose iterates
-                         // at that count are neither in st nor in a snapshot
-  int32_t mode;
-  double dt, sor1, sor2, tolrsp;
+  int32_t redo;          // 1: re-run from scratch, to each grid's redo count, the cells whose
+                         // iterates at that count are neither in st nor in a snapshot
 };
 
-// A segment of Gauss-Seidel sweeps (solvesp.f:87-2831) for every cell that has
-// not stopped: from Y (start = 0) or from the saved iterates, with the slab's
-// RC.  Sweeps are deterministic functions of the iterates and the slab, so a
-// segment continues the previous one bit for bit.
-// One wave per block; the block's dynamic LDS (UC_SWEEP_LDS doubles per lane) is
-// the sweep's second-level cache of slab values.
-__global__ void __launch_bounds__(WAVE) k_sweep(const UcParams* __restrict__ Pg, SweepArgs a) {
-  extern __shared__ double uc_lds[];
-  double* const lds = uc_lds + threadIdx.x;
-  UcP P = (UcP)Pg;
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = q < a.ncells;
-  const int64_t qq = valid ? q : a.ncells - 1;
-  const int64_t wave = q / WAVE;
-  const int32_t w0 = a.nsw[qq];
+// A segment of Gauss-Seidel sweeps (solvesp.f:87-2831) for one tile of 64 list
+// positions: every cell that has not stopped continues from Y (segment start 0) or
+// from its saved iterates, with the slab's RC.  Sweeps are deterministic functions
+// of the iterates and the slab, so a segment continues the previous one bit for bit.
+__device__ __forceinline__ void sweep_tile(UcP P, const SweepArgs& a, double* lds, int64_t tile) {
+  const int64_t q = tile * WAVE + (threadIdx.x & (WAVE - 1));
+  const int32_t g = wave_grid(a.gid, q);
+  const GridTab& G = a.gt[g];
+  const int32_t start = G.start, count = a.redo ? G.redo : G.count;
+  if (count <= 0) return;  // grid done (its residual rows are not read)
+  bool valid;
+  const int64_t c = list_cell(a.cells, q, valid);
+  const int32_t w0 = a.nsw[q];
   bool active;
-  if (a.redo > 0) {
+  if (a.redo) {
     const int32_t m = w0 & NSW_MASK;
     const bool fx = (w0 & NSW_FIXED) != 0;
-    active = valid && !(fx && m - 1 <= a.redo) && !(!fx && m == a.redo);
+    active = valid && !(fx && m - 1 <= G.redo) && !(!fx && m == G.redo);
   } else {
-    active = valid && (a.start == 0 || (w0 & NSW_FIXED) == 0);
+    active = valid && (start == 0 || (w0 & NSW_FIXED) == 0);
   }
-  if (!__any(active)) return;  // this wave's residual rows stay zero
-  const bool was_active = active;
-  if (a.redo > 0) {
+  const int32_t first = a.redo ? 0 : start;
+  if (a.redo) {
     const unsigned long long b = __ballot(active);
-    if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(a.redo_count, (int32_t)__popcll(b));
+    if ((threadIdx.x & (WAVE - 1)) == 0 && b) atomicAdd(&a.redo_count[g], (int32_t)__popcll(b));
   }
-  RcSlab rc = wave_slab(a.slabs);
-  const int64_t c = a.cells[qq];
+  if (!__any(active)) {  // residual rows of this tile: zero
+    if (a.rsd_part && (threadIdx.x & (WAVE - 1)) == 0)
+      for (int32_t sw = first; sw < first + count; ++sw) a.rsd_part[(int64_t)sw * a.ntiles + tile] = 0.0;
+    return;
+  }
+  const bool was_active = active;
+  RcSlab rc = tile_slab(a.slabs, tile);
   const double rho = a.rho[c], stmf = a.stmf[c], stnd = a.stnd[c];
+  const double dt = G.dt, sor1 = G.sor1, sor2 = G.sor2, tol = G.tolrsp;
+  const bool percell = G.mode == UC_MODE_PERCELL;
   double s[54];
   int32_t nsw;
-  if (a.start == 0) {  // SA = FA (spgsolv.f:169-220), soot from the field
+  if (first == 0) {  // SA = FA (spgsolv.f:169-220), soot from the field
     const FieldBuf Y = field_buf(a.y, a.n * 8 * NSP, c);
 #pragma unroll
     for (int k = 0; k < NSP; ++k) s[k] = Y.ld(k * a.n);
@@ -242,7 +269,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const UcParams* __restrict__ Pg,
     s[53] = stnd;
     nsw = 0;
   } else {
-    const FieldBuf ST = field_buf(a.st, a.npad * 8 * 54, qq);
+    const FieldBuf ST = field_buf(a.st, a.npad * 8 * 54, q);
 #pragma unroll
     for (int k = 0; k < 54; ++k) s[k] = ST.ld(k * a.npad);
     nsw = w0 & NSW_MASK;
@@ -250,37 +277,39 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const UcParams* __restrict__ Pg,
   bool fixed = false;
   double sst2_prev = 0.0;
   uint32_t executed = 0;
-  for (int32_t sw = a.start; sw < a.start + a.count; ++sw) {
-    if (!__any(active)) break;
+  const int32_t snap_from = G.snap_from;
+  for (int32_t sw = first; sw < first + count; ++sw) {
     double r = 0.0;
-    if (active) {
-      sst2_prev = s[53];
-      double d2;
-      r = uc_sweep(P, rc, lds, s, rho, stmf, stnd, a.dt, a.sor1, a.sor2, &d2);
-      ++nsw;
-      ++executed;
-      if (a.mode == UC_MODE_PERCELL) {
-        if (r / 51.0 <= a.tolrsp) { active = false; fixed = true; }
-      } else if (r == 0.0) {
-        active = false;  // every later sweep repeats this one exactly
-        fixed = true;
-      }
-      if (a.snap && sw + 1 >= a.snap_from) {
-        const FieldBuf SN = field_buf(a.snap + (int64_t)(sw + 1 - a.snap_from) * 54 * a.npad,
-                                      a.npad * 8 * 54, qq);
+    if (__any(active)) {
+      if (active) {
+        sst2_prev = s[53];
+        double d2;
+        r = uc_sweep(P, rc, lds, s, rho, stmf, stnd, dt, sor1, sor2, &d2);
+        ++nsw;
+        ++executed;
+        if (percell) {
+          if (r / 51.0 <= tol) { active = false; fixed = true; }
+        } else if (r == 0.0) {
+          active = false;  // every later sweep repeats this one exactly
+          fixed = true;
+        }
+        if (a.snap && sw + 1 >= snap_from) {
+          const FieldBuf SN = field_buf(a.snap + (int64_t)(sw + 1 - snap_from) * 54 * a.npad,
+                                        a.npad * 8 * 54, q);
 #pragma unroll
-        for (int k = 0; k < 54; ++k) SN.st(k * a.npad, s[k]);
+          for (int k = 0; k < 54; ++k) SN.st(k * a.npad, s[k]);
+        }
       }
     }
     if (a.rsd_part) {
       const double ws = wave_sum(r);
-      if ((threadIdx.x & (WAVE - 1)) == 0) a.rsd_part[(int64_t)sw * a.rsd_stride + wave] = ws;
+      if ((threadIdx.x & (WAVE - 1)) == 0) a.rsd_part[(int64_t)sw * a.ntiles + tile] = ws;
     }
   }
   {
     unsigned long long e = executed;
     for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, WAVE);
-    if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(a.swcount, e);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && e) atomicAdd(&a.swcount[g], e);
   }
   if (!was_active) return;
   const FieldBuf ST = field_buf(a.st, a.npad * 8 * 54, q);
@@ -290,19 +319,40 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const UcParams* __restrict__ Pg,
   if (fixed) a.sst2p[q] = sst2_prev;
 }
 
+// One wave per block; the block's dynamic LDS (UC_SWEEP_LDS doubles per lane) is
+// the sweep's second-level cache of slab values.  Tiles come from `order` (the
+// stiffness-sorted device work queue): one per block, or, in the persistent form,
+// as many as each wave takes from the shared queue position.
+template <bool PERSISTENT>
+__global__ void __launch_bounds__(WAVE) k_sweep(const UcParams* __restrict__ Pg, SweepArgs a) {
+  extern __shared__ double uc_lds[];
+  double* const lds = uc_lds + threadIdx.x;
+  UcP P = (UcP)Pg;
+  if (!PERSISTENT) {
+    const int64_t t = blockIdx.x;
+    sweep_tile(P, a, lds, a.order ? a.order[t] : t);
+    return;
+  }
+  for (;;) {
+    int32_t t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(a.queue, 1);
+    t = __builtin_amdgcn_readfirstlane(t);  // lane 0's ticket, uniform
+    if (t >= a.ntiles) return;  // every wave reaches the end of the queue
+    sweep_tile(P, a, lds, a.order ? __builtin_amdgcn_readfirstlane(a.order[t]) : t);
+    asm volatile("" : "+s"(P));
+  }
+}
+
 struct SourcesArgs {
   const double* rho; const double* tk; const double* ht; const double* stmf; const double* stnd;
-  const int32_t* cells; int32_t ncells; int64_t n; int64_t npad;
+  const int32_t* cells; const int32_t* gid; const GridTab* gt;
+  int64_t npad; int64_t n;
   const double* slabs; const double* st; const double* snap; const double* sst2p; const int32_t* nsw;
-  int32_t nstar;        // decided sweep count (global mode)
-  int32_t snap_from;    // sweep count of snapshot 0 of the last segment
   int32_t nsnap;
-  int32_t mode;
-  double dt, sor1, sor2;
   double* sa;           // [52][npad] species after the closure
   double* sst;          // [2][npad]  SST1, SST2
   double* rhsz;         // [npad]
-  double* qdot; double* qdsp;  // field arrays (solveh.f:1137-1142)
+  double* qdot; double* qdsp;  // packed field arrays (solveh.f:1137-1142)
 };
 
 // Iterates at the decided count, SST2 completion, N2 closure (solvesp.f:2839-2914)
@@ -310,30 +360,36 @@ struct SourcesArgs {
 __global__ void __launch_bounds__(256) k_sources(const UcParams* __restrict__ Pg, SourcesArgs a) {
   UcP P = (UcP)Pg;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= a.ncells) return;
-  RcSlab rc = wave_slab((double*)a.slabs);
-  const int64_t c = a.cells[q], n = a.n;
+  if (q >= a.npad) return;
+  bool valid;
+  const int64_t c = list_cell(a.cells, q, valid);
+  if (!valid) return;
+  const GridTab& G = a.gt[a.gid[q]];
+  RcSlab rc = tile_slab((double*)a.slabs, q / WAVE);
+  const int64_t n = a.n;
   const double rho = a.rho[c], tk = a.tk[c], ht = a.ht[c], stmf = a.stmf[c], stnd = a.stnd[c];
   const int32_t w = a.nsw[q];
   const int32_t m = w & NSW_MASK;
   const bool fx = (w & NSW_FIXED) != 0;
-  const bool global = a.mode != UC_MODE_PERCELL;
-  const bool use_st = !global || (fx && m - 1 <= a.nstar) || (!fx && m == a.nstar);
+  const bool global = G.mode != UC_MODE_PERCELL;
+  const int32_t nstar = G.nstar;
+  const bool use_st = !global || (fx && m - 1 <= nstar) || (!fx && m == nstar);
   const FieldBuf SRC = use_st ? field_buf(a.st, a.npad * 8 * 54, q)
-                              : field_buf(a.snap + (int64_t)(a.nstar - a.snap_from) * 54 * a.npad,
+                              : field_buf(a.snap + (int64_t)(nstar - G.snap_from) * 54 * a.npad,
                                           a.npad * 8 * 54, q);
+  const double dt = G.dt, sor1 = G.sor1, sor2 = G.sor2;
   double s[54];
 #pragma unroll
   for (int k = 0; k < 54; ++k) s[k] = SRC.ld(k * a.npad);
-  if (global && fx && m - 1 <= a.nstar) {
+  if (global && fx && m - 1 <= nstar) {
     // a fixed cell's species repeat; only SST2 keeps relaxing (solvesp.f:2820-2831)
-    if (a.nstar == m - 1) {
+    if (nstar == m - 1) {
       s[53] = a.sst2p[q];
     } else {
-      const double b = (rho * stnd + a.dt * rc.ld(UC_SLOT_SOUR + 1) * s[21]) * a.sor1
-                       / (rho + a.dt * rc.ld(UC_SLOT_SOUR + 5));
-      for (int32_t i = m; i < a.nstar; ++i) {
-        const double fnew = a.sor2 * s[53] + b;
+      const double b = (rho * stnd + dt * rc.ld(UC_SLOT_SOUR + 1) * s[21]) * sor1
+                       / (rho + dt * rc.ld(UC_SLOT_SOUR + 5));
+      for (int32_t i = m; i < nstar; ++i) {
+        const double fnew = sor2 * s[53] + b;
         s[53] = fnew < 0.0 ? 0.0 : fnew;
       }
     }
@@ -372,7 +428,7 @@ __global__ void __launch_bounds__(256) k_sources(const UcParams* __restrict__ Pg
   const double crad = P->alstr / P->rstr / P->vstr / P->vstr / P->vstr;
   const double qrad = crad * (double)0.4f * (4.0 * 5.669e-08 * plankc * (pow(T, 4.0) - 8.1e+09)
                                             + 3.334e-04 * stmf * (rho / 1.9e+03) * T5);
-  double rhsz = -qrad * a.dt;
+  double rhsz = -qrad * dt;
   const double rconv = P->vstr / P->wmstr / P->alstr * 1.0e-06;
   double prod = 0.0, hrr = 0.0;
   const FieldBuf QD = field_buf(a.qdsp, n * 8 * NSP, c);
@@ -390,7 +446,7 @@ __global__ void __launch_bounds__(256) k_sources(const UcParams* __restrict__ Pg
     hrr += prd * henth / Q->hfo[k];
     QD.st(k * n, -rconv * prd / Q->hfo[k]);
   });
-  rhsz = rhsz + rho * ht - a.dt * prod;
+  rhsz = rhsz + rho * ht - dt * prod;
   a.rhsz[q] = rhsz;
   a.qdot[c] = -hrr * rconv;
   const FieldBuf SA = field_buf(a.sa, a.npad * 8 * NSP, q);
@@ -400,38 +456,40 @@ __global__ void __launch_bounds__(256) k_sources(const UcParams* __restrict__ Pg
   a.sst[a.npad + q] = s[53];
 }
 
+
 struct HresArgs {
   const double* rho; const double* ht; const double* rhsz;
-  const int32_t* cells; int32_t ncells;
-  double* part;  // [isor2][nwaves]
-  int32_t nmax;
-  double zsor1, zsor2;
+  const int32_t* cells; const int32_t* gid; const GridTab* gt;
+  int64_t npad;
+  double* part;  // [H][ntiles] per-tile sums of enthalpy sweeps hfrom .. hfrom+H-1
+  int32_t H, ntiles;
 };
 
+// enthalpy relaxation residual trajectory of every cell (solveh.f:1147-1167):
+// sweeps G.hfrom .. G.hfrom+H-1 of its grid, per-tile sums (zero-transport: PPP = RHO)
 __global__ void __launch_bounds__(256) k_hres(HresArgs a) {
-  const int64_t G = (int64_t)gridDim.x * blockDim.x;
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int wave = (int)(tid / WAVE), nwaves = (int)(G / WAVE);
-  const int64_t iters = (a.ncells + G - 1) / G;
-  for (int64_t it = 0; it < iters; ++it) {
-    const int64_t q = it * G + tid;
-    const bool valid = q < a.ncells;
-    const int64_t qq = valid ? q : 0;
-    const int64_t c = a.cells[qq];
-    double fpz = a.ht[c];
-    const double inc = a.rhsz[qq] * a.zsor1 / a.rho[c];
-    bool active = valid;
-    for (int sw = 0; sw < a.nmax; ++sw) {
-      if (!__any(active)) break;
-      double d = 0.0;
-      if (active) {
-        const double fnew = a.zsor2 * fpz + inc;
-        d = fabs(fpz - fnew);
-        fpz = fnew;
-        if (d == 0.0) active = false;
-      }
-      const double wsum = wave_sum(d);
-      if ((threadIdx.x & (WAVE - 1)) == 0) a.part[(int64_t)sw * nwaves + wave] += wsum;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.npad) return;
+  const int64_t tile = q / WAVE;
+  const GridTab& G = a.gt[wave_grid(a.gid, q)];
+  if (G.hfrom < 0) return;  // grid decided
+  bool valid;
+  const int64_t c = list_cell(a.cells, q, valid);
+  double fpz = a.ht[c];
+  const double inc = a.rhsz[q] * G.zsor1 / a.rho[c];
+  const double zsor2 = G.zsor2;
+  bool active = valid;
+  for (int32_t sw = 0; sw < G.hfrom + a.H; ++sw) {
+    double d = 0.0;
+    if (active) {
+      const double fnew = zsor2 * fpz + inc;
+      d = fabs(fpz - fnew);
+      fpz = fnew;
+      if (d == 0.0) active = false;
+    }
+    if (sw >= G.hfrom) {
+      const double ws = wave_sum(d);
+      if ((threadIdx.x & (WAVE - 1)) == 0) a.part[(int64_t)(sw - G.hfrom) * a.ntiles + tile] = ws;
     }
   }
 }
@@ -439,20 +497,23 @@ __global__ void __launch_bounds__(256) k_hres(HresArgs a) {
 struct FinishArgs {
   double* rho; double* y; double* tk; double* ht; double* stmf; double* stnd;
   const double* sa; const double* sst; const double* rhsz;  // [52][npad], [2][npad], [npad]
-  const int32_t* cells; int32_t ncells; int64_t n; int64_t npad;
-  int32_t nh;        // decided enthalpy sweep count (global mode) / cap (per-cell)
-  int32_t mode;
-  double zsor1, zsor2, tolrh;
-  double* res_part;  // [2][nblocks] RESDC, RESDH partial sums
+  const int32_t* cells; const int32_t* gid; const GridTab* gt;
+  int64_t npad; int64_t n;
+  double* res_part;  // [2][ntiles] per-tile RESDC, RESDH partial sums
+  int32_t ntiles;
   double* fpz_out;   // null: SPGSOLV epilogue + density; else FPZ here and TK only
 };
 
 __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg, FinishArgs a) {
   UcP P = (UcP)Pg;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.npad) return;
+  bool valid;
+  const int64_t c = list_cell(a.cells, q, valid);
+  const GridTab& G = a.gt[wave_grid(a.gid, q)];
   double rc_ = 0.0, rh_ = 0.0;
-  if (q < a.ncells) {
-    const int64_t c = a.cells[q], n = a.n;
+  if (valid) {
+    const int64_t n = a.n;
     const FieldBuf SA = field_buf(a.sa, a.npad * 8 * NSP, q), Y = field_buf(a.y, n * 8 * NSP, c);
     double s[NSP];
 #pragma unroll
@@ -460,12 +521,13 @@ __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg,
     const double rho = a.rho[c], ht = a.ht[c];
     // enthalpy relaxation (solveh.f:1147-1167), zero transport: PPP = RHO
     double fpz = ht;
-    const double inc = a.rhsz[q] * a.zsor1 / rho;
-    for (int sw = 0; sw < a.nh; ++sw) {
-      const double fnew = a.zsor2 * fpz + inc;
+    const double inc = a.rhsz[q] * G.zsor1 / rho;
+    const bool percell = G.mode == UC_MODE_PERCELL;
+    for (int32_t sw = 0; sw < G.nh; ++sw) {
+      const double fnew = G.zsor2 * fpz + inc;
       const double d = fabs(fpz - fnew);
       fpz = fnew;
-      if (a.mode == UC_MODE_PERCELL && d <= a.tolrh) break;
+      if (percell && d <= G.tolrh) break;
     }
     // T from h (solveh.f:1170-1230)
     double enth = fpz * P->hstr;
@@ -509,7 +571,7 @@ __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg,
     if (a.fpz_out) {  // SOLVEH's outputs only (solveh.f:1167, :1230)
       a.fpz_out[c] = fpz;
       a.tk[c] = tnew;
-      goto reduce;
+      goto done;
     }
     {
     // SPGSOLV epilogue (spgsolv.f:1299-1416)
@@ -530,20 +592,12 @@ __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg,
     a.rho[c] = P->pback / (P->beta1 * tnew * rbar);  // main:1056-1061
     }
   }
-reduce:
-  __shared__ double red[2][256 / WAVE];
+done:
   rc_ = wave_sum(rc_);
   rh_ = wave_sum(rh_);
   if ((threadIdx.x & (WAVE - 1)) == 0) {
-    red[0][threadIdx.x / WAVE] = rc_;
-    red[1][threadIdx.x / WAVE] = rh_;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double x = 0.0, z = 0.0;
-    for (int w = 0; w < (int)(blockDim.x / WAVE); ++w) { x += red[0][w]; z += red[1][w]; }
-    a.res_part[blockIdx.x] = x;
-    a.res_part[gridDim.x + blockIdx.x] = z;
+    a.res_part[q / WAVE] = rc_;
+    a.res_part[a.ntiles + q / WAVE] = rh_;
   }
 }
 
@@ -557,21 +611,110 @@ __global__ void k_density(const UcParams* __restrict__ P, double* __restrict__ r
   rho[c] = P->pback / (P->beta1 * tk[c] * rbar);
 }
 
-// main:1062-1068: last I column extrapolated, then the J=1 row copied from J=2
-__global__ void k_density_edge(double* __restrict__ rho, int32_t li, int32_t lj) {
-  const int t = threadIdx.x;
-  const int64_t n = (int64_t)li * lj;
-  for (int j = t; j < lj; j += blockDim.x) {
+// main:1062-1068, per grid (block g): last I column extrapolated, then the J=1 row
+// copied from J=2
+__global__ void k_density_edge(double* __restrict__ rho, const GridTab* __restrict__ gt) {
+  const GridTab& G = gt[blockIdx.x];
+  double* r = rho + G.off;
+  const int64_t li = G.li, lj = G.lj, n = li * lj;
+  for (int64_t j = threadIdx.x; j < lj; j += blockDim.x) {
     const int64_t c = n - lj + j;
-    rho[c] = 2.0 * rho[c - lj] - rho[c - 2 * lj];
+    r[c] = 2.0 * r[c - lj] - r[c - 2 * lj];
   }
   __syncthreads();
-  for (int i = t; i < li; i += blockDim.x) rho[(int64_t)i * lj] = rho[(int64_t)i * lj + 1];
+  for (int64_t i = threadIdx.x; i < li; i += blockDim.x) r[i * lj] = r[i * lj + 1];
 }
 
 __global__ void k_zero(double* __restrict__ p, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
+}
+
+// ---- batches: per-grid field arrays <-> the packed arrays (grid g at offset off_g)
+struct PackPtrs {
+  double *rho, *y, *tk, *ht, *stmf, *stnd, *qdot, *qdsp;
+  const int32_t* iskip;
+};
+
+// block (x, g): cells x*256.. of grid g; dir 0 packs (grid -> packed), 1 unpacks
+__global__ void k_pack(const PackPtrs* __restrict__ grids, const GridTab* __restrict__ gt, PackPtrs pk,
+                       int32_t* __restrict__ pk_iskip, int64_t N, int dir) {
+  const int32_t g = blockIdx.y;
+  const GridTab& G = gt[g];
+  const int64_t ng = (int64_t)G.li * G.lj;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ng) return;
+  const PackPtrs& f = grids[g];
+  const int64_t d = G.off + c;
+  if (dir == 0) {
+    pk.rho[d] = f.rho[c]; pk.tk[d] = f.tk[c]; pk.ht[d] = f.ht[c];
+    pk.stmf[d] = f.stmf[c]; pk.stnd[d] = f.stnd[c]; pk_iskip[d] = f.iskip[c];
+    for (int k = 0; k < NSP; ++k) pk.y[k * N + d] = f.y[k * ng + c];
+  } else {
+    f.rho[c] = pk.rho[d]; f.tk[c] = pk.tk[d]; f.ht[c] = pk.ht[d];
+    f.stmf[c] = pk.stmf[d]; f.stnd[c] = pk.stnd[d]; f.qdot[c] = pk.qdot[d];
+    for (int k = 0; k < NSP; ++k) {
+      f.y[k * ng + c] = pk.y[k * N + d];
+      f.qdsp[k * ng + c] = pk.qdsp[k * N + d];
+    }
+  }
+}
+
+// ---- stiffness binning: sort keys (grid, bin of the previous step's sweeps)
+__global__ void k_bin_keys(const int32_t* __restrict__ base, const int32_t* __restrict__ base_gid,
+                           const int32_t* __restrict__ prev_nsw, int32_t nbase, int32_t bins, int32_t maxsw,
+                           uint32_t* __restrict__ keys, int32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbase) return;
+  const int32_t sw = prev_nsw[base[i]];
+  int32_t b = (int32_t)((int64_t)sw * bins / (maxsw + 1));
+  b = b < 0 ? 0 : (b >= bins ? bins - 1 : b);
+  keys[i] = ((uint32_t)base_gid[i] << 6) | (uint32_t)(bins - 1 - b);  // stiffest first within a grid
+  vals[i] = base[i];
+}
+
+// sorted (grid-major) cells into the padded list: grid g's cells start at tile0*64,
+// padding lanes repeat the grid's first cell as -1-c
+__global__ void k_bin_place(const int32_t* __restrict__ sorted_cells, const int32_t* __restrict__ base_off,
+                            const GridTab* __restrict__ gt, int32_t ngrids, int32_t* __restrict__ cells,
+                            int32_t* __restrict__ gid, int64_t npad) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npad) return;
+  // the grid of list position q: binary search on tile0
+  int32_t lo = 0, hi = ngrids - 1;
+  const int32_t t = (int32_t)(q / WAVE);
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) / 2;
+    if (gt[mid].tile0 <= t) lo = mid; else hi = mid - 1;
+  }
+  const int32_t g = lo;
+  const int64_t r = q - (int64_t)gt[g].tile0 * WAVE;       // rank within the grid
+  const int64_t cnt = base_off[g + 1] - base_off[g];
+  gid[q] = g;
+  cells[q] = r < cnt ? sorted_cells[base_off[g] + r] : -1 - sorted_cells[base_off[g]];
+}
+
+// after a step: each cell's sweeps, by field cell, for the next step's bins
+__global__ void k_save_nsw(const int32_t* __restrict__ cells, const int32_t* __restrict__ nsw, int64_t npad,
+                           int32_t* __restrict__ prev_nsw) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npad) return;
+  const int32_t v = cells[q];
+  if (v >= 0) prev_nsw[v] = nsw[q] & NSW_MASK;
+}
+
+// tile costs (stiffest bin of the tile) for the work-queue order
+__global__ void k_tile_keys(const int32_t* __restrict__ cells, const int32_t* __restrict__ prev_nsw,
+                            int32_t ntiles, uint32_t* __restrict__ keys, int32_t* __restrict__ vals) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  int32_t m = 0;
+  for (int l = 0; l < WAVE; ++l) {
+    const int32_t v = cells[t * WAVE + l];
+    if (v >= 0) m = max(m, prev_nsw[v]);
+  }
+  keys[t] = 0x7fffffffu - (uint32_t)m;  // descending cost
+  vals[t] = (int32_t)t;
 }
 
 }  // namespace
@@ -580,16 +723,35 @@ __global__ void k_zero(double* __restrict__ p, int64_t n) {
 namespace {
 // bookkeeping of the step in progress (chem_host.inc: ph_begin .. ph_end)
 struct StepRun {
-  const uc_field* f = nullptr;
-  uc_controls ctl{};
   hipStream_t s = nullptr;
-  int64_t n = 0, npad = 0, nwc = 0, nb = 0;
-  int32_t ncells = 0, nstar = 0, nh = 0, snap_from = 1;
-  bool global = true;
-  double sor1 = 0.0, sor2 = 0.0, spnorm = 0.0, hnorm = 0.0;
+  int32_t ngrids = 0;
+  bool clip = false;
   float ms_rates = 0.f, ms_sweeps = 0.f, ms_sources = 0.f;
-  uc_stats out{};
-  std::vector<double> r = std::vector<double>(1024, 0.0);  // per-sweep residual sums
+  int32_t sweep_launches = 0;
+  std::vector<uc_field> fields;       // the caller's grids
+  std::vector<uc_controls> ctl;       // their controls
+  std::vector<double> spnorm, hnorm;  // global-stop norms per grid
+  std::vector<uc_stats> out;          // per-grid statistics
+};
+
+// the cell list of the current grids (rebuilt when the grids change)
+struct Layout {
+  int32_t ngrids = 0;
+  std::vector<const void*> key;  // per grid: ISKIP pointer, li, lj
+  int64_t N = 0;                 // packed cells
+  int64_t npad = 0;              // list positions (grids padded to whole tiles)
+  int32_t ntiles = 0;
+  int32_t nbase = 0;             // active cells
+  bool packed = false;           // batch: grids copied to the context's packed arrays
+  bool have_prev = false;        // prev_nsw holds the previous step's sweeps
+};
+
+// tuning of a step: the autotuner's choices (uc_tuning)
+struct Tuning {
+  int32_t bins = 1;        // stiffness bins of the cell order (1: natural order)
+  int32_t lpt = 0;         // tiles stiffest first
+  int32_t persistent = 1;  // k_sweep as persistent waves over the tile queue
+  int32_t block = 256;     // block size of the one-thread-per-cell kernels
 };
 }  // namespace
 
@@ -597,9 +759,11 @@ struct uc_context {
   uc_scales sc{};
   UcParams hp{};
   UcParams* dp = nullptr;
-  int64_t max_cells = 0;
-  int64_t max_pad = 0;     // max_cells rounded up to whole waves
+  int64_t max_cells = 0;   // packed cells over all grids
+  int32_t max_grids = 1;
+  int64_t max_pad = 0;     // list positions: max_cells + 64 per grid
   int device = 0;
+  int cus = 0;
   int32_t nsnap = 0;       // snapshot slots per cell
   double* slabs = nullptr; // [max_pad/64][SLAB_SLOTS][64]
   double* st = nullptr;    // [54][max_pad]
@@ -609,22 +773,35 @@ struct uc_context {
   double* sst = nullptr;   // [2][max_pad]
   double* rhsz = nullptr;  // [max_pad]
   int32_t* nsw = nullptr;  // [max_pad]
-  int32_t* cells = nullptr;
-  uint8_t* done = nullptr;
-  double* rsd_part = nullptr;  // per-sweep, per-wave residual sums
-  int64_t rsd_part_len = 0;
-  int64_t hres_threads = 0;    // k_hres grid (persistent)
-  double* res_part = nullptr;
-  int64_t res_part_len = 0;
-  double* rows = nullptr;      // per-sweep residual sums (<= 1024 sweeps)
-  unsigned long long* counters = nullptr;  // [0] sweeps executed, [1] (int) cells re-run
+  int32_t* cells = nullptr;  // [max_pad] packed field index, or -1-c for padding
+  int32_t* gid = nullptr;    // [max_pad] grid of each list position
+  int32_t* base = nullptr;   // [max_cells] active cells, grid-major natural order
+  int32_t* base_gid = nullptr;
+  int32_t* base_off = nullptr;  // [max_grids+1]
+  int32_t* base_list = nullptr; // [max_pad] the natural-order padded list (bins off)
+  int32_t* base_list_gid = nullptr;
+  int32_t* prev_nsw = nullptr;  // [max_cells] previous step's sweeps per packed cell
+  uint32_t* keys[2]{};          // radix sort buffers [max_cells]
+  int32_t* vals[2]{};
+  int32_t* order = nullptr;     // [max_tiles] tile order (the work queue)
+  int32_t* queue = nullptr;     // persistent k_sweep: next queue position
+  void* sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  uint8_t* done = nullptr;      // [max_cells]
+  GridTab* gt_dev = nullptr;    // [max_grids]
+  GridTab* gt = nullptr;        // pinned host mirror
+  PackPtrs* grid_ptrs = nullptr;  // [max_grids] device copy of the grids' pointers (batches)
+  PackPtrs pk{};                // packed field arrays (batches; allocated on first use)
+  int32_t* pk_iskip = nullptr;
+  double* rsd_part = nullptr;   // [128][max_tiles] per-sweep per-tile residual sums
+  double* res_part = nullptr;   // [2][max_tiles]
+  double* rows = nullptr;       // [max_grids][128] row sums
+  double* rows_h = nullptr;     // pinned host copy
+  unsigned long long* counters = nullptr;  // [2][max_grids] sweeps executed, cells re-run
   hipEvent_t ev[8]{};
-  double last_dt = -1.0;
-  int32_t last_nstar = 0;  // previous step's species sweep count (first segment length)
-  // cell list cache
-  const int32_t* mask_ptr = nullptr;
-  int32_t mask_li = 0, mask_lj = 0;
-  std::vector<int32_t> cell_list;
+  std::vector<int32_t> last_nstar;  // per grid: previous step's species sweep count
+  Layout lay;
+  Tuning tune;
   StepRun run;
 };
 
@@ -652,11 +829,12 @@ int uc_scales_from_card(double vref, double tref, double pref, double rtin, doub
   return UC_OK;
 }
 
-int uc_create(const uc_scales* scales, int64_t max_cells, uc_context** out) {
-  if (!scales || !out || max_cells < 9) return fail(UC_E_ARG, "uc_create: bad arguments");
+int uc_create_batch(const uc_scales* scales, int64_t max_cells, int32_t max_grids, uc_context** out) {
+  if (!scales || !out || max_cells < 9 || max_grids < 1 || max_grids > (1 << 20))
+    return fail(UC_E_ARG, "uc_create: bad arguments");
   // buffer instructions address each per-cell array with 32-bit byte offsets
-  if (max_cells > UC_MAX_CELLS)
-    return fail(UC_E_ARG, "uc_create: max_cells above " + std::to_string(UC_MAX_CELLS));
+  if (max_cells + (int64_t)WAVE * max_grids > UC_MAX_CELLS)
+    return fail(UC_E_ARG, "uc_create: max_cells + 64*max_grids above " + std::to_string(UC_MAX_CELLS));
   *out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(UC_E_NOGPU, "uc_create: no HIP device");
@@ -666,12 +844,14 @@ int uc_create(const uc_scales* scales, int64_t max_cells, uc_context** out) {
   HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
   if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0)
     return fail(UC_E_NOGPU, std::string("uc_create: device is ") + prop.gcnArchName + ", built for gfx950");
+  ctx->cus = prop.multiProcessorCount;
   ctx->sc = *scales;
   fill_params(ctx->sc, ctx->hp);
   ctx->max_cells = max_cells;
-  ctx->max_pad = (max_cells + WAVE - 1) / WAVE * WAVE;
+  ctx->max_grids = max_grids;
+  ctx->max_pad = (max_cells + (int64_t)WAVE * max_grids + WAVE - 1) / WAVE * WAVE;
   ctx->nsnap = 8;
-  const int64_t mp = ctx->max_pad;
+  const int64_t mp = ctx->max_pad, mt = mp / WAVE;
   HIPCHK(hipMalloc(&ctx->dp, sizeof(UcParams)));
   HIPCHK(hipMemcpy(ctx->dp, &ctx->hp, sizeof(UcParams), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&ctx->slabs, sizeof(double) * SLAB_SLOTS * mp));
@@ -682,30 +862,61 @@ int uc_create(const uc_scales* scales, int64_t max_cells, uc_context** out) {
   HIPCHK(hipMalloc(&ctx->sst, sizeof(double) * 2 * mp));
   HIPCHK(hipMalloc(&ctx->rhsz, sizeof(double) * mp));
   HIPCHK(hipMalloc(&ctx->nsw, sizeof(int32_t) * mp));
-  HIPCHK(hipMalloc(&ctx->cells, sizeof(int32_t) * max_cells));
+  HIPCHK(hipMalloc(&ctx->cells, sizeof(int32_t) * mp));
+  HIPCHK(hipMalloc(&ctx->gid, sizeof(int32_t) * mp));
+  HIPCHK(hipMalloc(&ctx->base_list, sizeof(int32_t) * mp));
+  HIPCHK(hipMalloc(&ctx->base_list_gid, sizeof(int32_t) * mp));
+  HIPCHK(hipMalloc(&ctx->base, sizeof(int32_t) * max_cells));
+  HIPCHK(hipMalloc(&ctx->base_gid, sizeof(int32_t) * max_cells));
+  HIPCHK(hipMalloc(&ctx->base_off, sizeof(int32_t) * (max_grids + 1)));
+  HIPCHK(hipMalloc(&ctx->prev_nsw, sizeof(int32_t) * max_cells));
+  for (int i = 0; i < 2; ++i) {
+    HIPCHK(hipMalloc(&ctx->keys[i], sizeof(uint32_t) * std::max(max_cells, mt)));
+    HIPCHK(hipMalloc(&ctx->vals[i], sizeof(int32_t) * std::max(max_cells, mt)));
+  }
+  HIPCHK(hipMalloc(&ctx->order, sizeof(int32_t) * mt));
+  HIPCHK(hipMalloc(&ctx->queue, sizeof(int32_t) * 64));
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, ctx->sort_tmp_bytes, ctx->keys[0], ctx->keys[1], ctx->vals[0],
+                                            ctx->vals[1], (int)std::max(max_cells, mt), 0, 32));
+  HIPCHK(hipMalloc(&ctx->sort_tmp, std::max<size_t>(ctx->sort_tmp_bytes, 256)));
   HIPCHK(hipMalloc(&ctx->done, max_cells));
-  ctx->hres_threads = std::min<int64_t>(mp, (int64_t)prop.multiProcessorCount * 1024);
-  ctx->rsd_part_len = std::max<int64_t>(128 * (mp / WAVE), 1000 * (ctx->hres_threads / WAVE));
-  HIPCHK(hipMalloc(&ctx->rsd_part, sizeof(double) * ctx->rsd_part_len));
-  ctx->res_part_len = 2 * ((max_cells + 255) / 256 + 1);
-  HIPCHK(hipMalloc(&ctx->res_part, sizeof(double) * ctx->res_part_len));
-  HIPCHK(hipMalloc(&ctx->rows, sizeof(double) * 1024));
-  HIPCHK(hipMalloc(&ctx->counters, sizeof(unsigned long long) * 2));
-  if (UC_SWEEP_LDS * 512 > 65536)
-    HIPCHK(hipFuncSetAttribute((const void*)k_sweep, hipFuncAttributeMaxDynamicSharedMemorySize,
+  HIPCHK(hipMalloc(&ctx->gt_dev, sizeof(GridTab) * max_grids));
+  HIPCHK(hipHostMalloc(&ctx->gt, sizeof(GridTab) * max_grids, hipHostMallocDefault));
+  HIPCHK(hipMalloc(&ctx->grid_ptrs, sizeof(PackPtrs) * max_grids));
+  HIPCHK(hipMalloc(&ctx->rsd_part, sizeof(double) * 128 * mt));
+  HIPCHK(hipMalloc(&ctx->res_part, sizeof(double) * 2 * mt));
+  HIPCHK(hipMalloc(&ctx->rows, sizeof(double) * 128 * max_grids));
+  HIPCHK(hipHostMalloc(&ctx->rows_h, sizeof(double) * 128 * max_grids, hipHostMallocDefault));
+  HIPCHK(hipMalloc(&ctx->counters, sizeof(unsigned long long) * 2 * max_grids));
+  if (UC_SWEEP_LDS * 512 > 65536) {
+    HIPCHK(hipFuncSetAttribute((const void*)k_sweep<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                UC_SWEEP_LDS * 512));
+    HIPCHK(hipFuncSetAttribute((const void*)k_sweep<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               UC_SWEEP_LDS * 512));
+  }
   for (auto& e : ctx->ev) HIPCHK(hipEventCreate(&e));
   *out = ctx.release();
   return UC_OK;
+}
+
+int uc_create(const uc_scales* scales, int64_t max_cells, uc_context** out) {
+  return uc_create_batch(scales, max_cells, 1, out);
 }
 
 int uc_destroy(uc_context* ctx) {
   if (!ctx) return UC_OK;
   for (void* p : {(void*)ctx->dp, (void*)ctx->slabs, (void*)ctx->st, (void*)ctx->snap, (void*)ctx->sst2p,
                   (void*)ctx->sa, (void*)ctx->sst, (void*)ctx->rhsz, (void*)ctx->nsw, (void*)ctx->cells,
-                  (void*)ctx->done, (void*)ctx->rsd_part, (void*)ctx->res_part, (void*)ctx->rows,
-                  (void*)ctx->counters})
-    (void)hipFree(p);
+                  (void*)ctx->gid, (void*)ctx->base_list, (void*)ctx->base_list_gid, (void*)ctx->base,
+                  (void*)ctx->base_gid, (void*)ctx->base_off, (void*)ctx->prev_nsw, (void*)ctx->keys[0],
+                  (void*)ctx->keys[1], (void*)ctx->vals[0], (void*)ctx->vals[1], (void*)ctx->order,
+                  (void*)ctx->queue, ctx->sort_tmp, (void*)ctx->done, (void*)ctx->gt_dev, (void*)ctx->grid_ptrs,
+                  (void*)ctx->rsd_part, (void*)ctx->res_part, (void*)ctx->rows, (void*)ctx->counters,
+                  (void*)ctx->pk.rho, (void*)ctx->pk.y, (void*)ctx->pk.tk, (void*)ctx->pk.ht, (void*)ctx->pk.stmf,
+                  (void*)ctx->pk.stnd, (void*)ctx->pk.qdot, (void*)ctx->pk.qdsp, (void*)ctx->pk_iskip})
+    if (p) (void)hipFree(p);
+  if (ctx->gt) (void)hipHostFree(ctx->gt);
+  if (ctx->rows_h) (void)hipHostFree(ctx->rows_h);
   for (auto& e : ctx->ev) (void)hipEventDestroy(e);
   delete ctx;
   return UC_OK;
@@ -715,8 +926,32 @@ int uc_step(uc_context* ctx, const uc_field* f, const uc_controls* ctl, void* st
   return step_impl(ctx, f, ctl, (hipStream_t)stream, st);
 }
 
+int uc_step_batch(uc_context* ctx, const uc_field* fields, const uc_controls* ctls, int32_t ngrids, void* stream,
+                  uc_stats* stats) {
+  return step_batch_impl(ctx, fields, ctls, ngrids, (hipStream_t)stream, stats);
+}
+
 int uc_step_host(uc_context* ctx, const uc_field* hf, const uc_controls* ctl, uc_stats* st) {
   return step_host_impl(ctx, hf, ctl, st);
+}
+
+int uc_autotune(uc_context* ctx, const uc_field* fields, const uc_controls* ctls, int32_t ngrids, void* stream,
+                uc_tuning* chosen) {
+  return autotune_impl(ctx, fields, ctls, ngrids, (hipStream_t)stream, chosen);
+}
+
+int uc_set_tuning(uc_context* ctx, const uc_tuning* t) {
+  if (!ctx || !t) return fail(UC_E_ARG, "uc_set_tuning: null argument");
+  if (t->bins < 1 || t->bins > 32 || (t->block != 64 && t->block != 128 && t->block != 256))
+    return fail(UC_E_ARG, "uc_set_tuning: bins must be 1..32 and block 64, 128 or 256");
+  ctx->tune = Tuning{t->bins, t->lpt ? 1 : 0, t->persistent ? 1 : 0, t->block};
+  return UC_OK;
+}
+
+int uc_get_tuning(const uc_context* ctx, uc_tuning* t) {
+  if (!ctx || !t) return fail(UC_E_ARG, "uc_get_tuning: null argument");
+  *t = uc_tuning{ctx->tune.bins, ctx->tune.lpt, ctx->tune.persistent, ctx->tune.block};
+  return UC_OK;
 }
 
 }  // extern "C"

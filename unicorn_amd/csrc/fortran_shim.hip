@@ -66,10 +66,11 @@ namespace {
 
 // k_finish and k_sources write per-list-position rows; scatter them to field layout
 __global__ void k_scatter_rows(const double* __restrict__ src, int64_t npad, const int32_t* __restrict__ cells,
-                               int32_t ncells, int32_t rows, double* __restrict__ dst, int64_t n) {
+                               int32_t rows, double* __restrict__ dst, int64_t n) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ncells) return;
-  const int64_t c = cells[q];
+  if (q >= npad) return;
+  const int32_t c = cells[q];
+  if (c < 0) return;  // padding lane
   for (int r = 0; r < rows; ++r) dst[r * n + c] = src[r * npad + q];
 }
 
@@ -185,9 +186,10 @@ void ckrates_(void) {
   h2d(g.qdot, hrls_.qdot, b1, "ckrates_");
   h2d(g.qdsp, dumd_.qdsp, b52, "ckrates_");
   hipcheck(hipStreamSynchronize(g.s), "ckrates_");
-  g.ctx->mask_ptr = nullptr;  // ISKIP may change between steps
-  uc_controls ctl{cb03_.dt, 0.99, 1e-8, 0.99, 1e-8, 100, 100, UC_MODE_GLOBAL, 0};
-  check(ph_begin(g.ctx, &g.f, &ctl, g.s, false, (double)shim::LE * 51.0, (double)shim::LE), "ckrates_");
+  g.ctx->lay.ngrids = 0;  // ISKIP may change between steps: rebuild the cell list
+  // SOLVESP/SOLVEH supply the controls and norms; these are placeholders until then
+  const uc_controls ctl{cb03_.dt, 0.99, 1e-8, 0.99, 1e-8, 100, 100, UC_MODE_GLOBAL, 0};
+  check(ph_begin(g.ctx, &g.f, &ctl, 1, g.s, false, nullptr, nullptr), "ckrates_");
   check(ph_rates(g.ctx), "ckrates_");
   g.stage = 1;
 }
@@ -198,21 +200,23 @@ void solvesp_(int32_t* isor1, double* relxsp, double* tolrsp, double* spnorm) {
   StepRun& R = g.ctx->run;
   if (*isor1 < 1 || *isor1 > 127) stop("solvesp_", "ISOR1 must be 1..127");
   if (!(*relxsp > 0.0 && *relxsp <= 1.0) || !(*spnorm > 0.0)) stop("solvesp_", "RELXSP or SPNORM out of range");
-  R.ctl.isor1 = *isor1;
-  R.ctl.relxsp = *relxsp;
-  R.ctl.tolrsp = *tolrsp;
-  R.sor1 = *relxsp;
-  R.sor2 = 1.0 - *relxsp;
-  R.spnorm = *spnorm;
-  R.nstar = *isor1;
+  R.ctl[0].isor1 = *isor1;
+  R.ctl[0].relxsp = *relxsp;
+  R.ctl[0].tolrsp = *tolrsp;
+  R.spnorm[0] = *spnorm;
+  GridTab& G = g.ctx->gt[0];
+  G.sor1 = *relxsp;
+  G.sor2 = 1.0 - *relxsp;
+  G.tolrsp = *tolrsp;
+  G.isor1 = *isor1;
   check(ph_species(g.ctx), "solvesp_");
-  *isor1 = R.out.isor1;
-  const int64_t nb = blocks_for(R.ncells, 256);
-  k_scatter_rows<<<nb, 256, 0, g.s>>>(g.ctx->sa, R.npad, g.ctx->cells, R.ncells, shim::LSP, g.ssp, shim::LE);
-  k_scatter_rows<<<nb, 256, 0, g.s>>>(g.ctx->sst, R.npad, g.ctx->cells, R.ncells, 2, g.sst, shim::LE);
+  *isor1 = R.out[0].isor1;
+  const int64_t npad = g.ctx->lay.npad, nb = blocks_for(npad, 256);
+  k_scatter_rows<<<nb, 256, 0, g.s>>>(g.ctx->sa, npad, g.ctx->cells, shim::LSP, g.ssp, shim::LE);
+  k_scatter_rows<<<nb, 256, 0, g.s>>>(g.ctx->sst, npad, g.ctx->cells, 2, g.sst, shim::LE);
   // the closure also sets the old-level N2, FCB = SCB (solvesp.f:2912)
-  k_scatter_rows<<<nb, 256, 0, g.s>>>(g.ctx->sa + 51 * R.npad, R.npad, g.ctx->cells, R.ncells, 1,
-                                      g.y + 51 * shim::LE, shim::LE);
+  k_scatter_rows<<<nb, 256, 0, g.s>>>(g.ctx->sa + 51 * npad, npad, g.ctx->cells, 1, g.y + 51 * shim::LE,
+                                      shim::LE);
   hipcheck(hipGetLastError(), "solvesp_");
   const size_t b1 = sizeof(double) * shim::LE;
   d2h(dumy_.ssp, g.ssp, b1 * shim::LSP, "solvesp_");
@@ -231,11 +235,15 @@ void solveh_(int32_t* isor2, double* relxh, double* tolrh, double* sigsp, double
   StepRun& R = g.ctx->run;
   if (*isor2 < 1 || *isor2 > 1000) stop("solveh_", "ISOR2 must be 1..1000");
   if (!(*relxh > 0.0 && *relxh <= 1.0) || !(*hnorm > 0.0)) stop("solveh_", "RELXH or HNORM out of range");
-  R.ctl.isor2 = *isor2;
-  R.ctl.relxh = *relxh;
-  R.ctl.tolrh = *tolrh;
-  R.hnorm = *hnorm;
-  R.nh = *isor2;
+  R.ctl[0].isor2 = *isor2;
+  R.ctl[0].relxh = *relxh;
+  R.ctl[0].tolrh = *tolrh;
+  R.hnorm[0] = *hnorm;
+  GridTab& G = g.ctx->gt[0];
+  G.zsor1 = *relxh;
+  G.zsor2 = 1.0 - *relxh;
+  G.tolrh = *tolrh;
+  G.isor2 = *isor2;
   check(ph_enthalpy(g.ctx), "solveh_");
   check(ph_finish(g.ctx, g.fpz), "solveh_");
   uc_stats st{};

@@ -212,7 +212,7 @@ SLOT_BASE = {"UC_SLOT_RHOFA": 544, "UC_SLOT_G": 595, "UC_SLOT_SOUR": 647, "UC_SL
 RC_READ = re.compile(r"rc\.ld\(([^()]*)\)")
 
 # register cache and prefetch depth of the sweep's slab reads (doubles)
-SWEEP_CACHE = int(os.environ.get("UC_SWEEP_CACHE", "100"))
+SWEEP_CACHE = int(os.environ.get("UC_SWEEP_CACHE", "70"))
 SWEEP_PREFETCH = int(os.environ.get("UC_SWEEP_PREFETCH", "32"))
 SWEEP_LDS = int(os.environ.get("UC_SWEEP_LDS", "126"))     # LDS positions (doubles per lane)
 
@@ -422,7 +422,7 @@ def emit_sweep(t, L):
     L.append("// s[0..51]: species iterates, s[52] soot mass fraction, s[53] soot number density.")
     L.append("// Returns sum |S - FNEW| over species 1..51 and the soot mass fraction (RSD's")
     L.append("// per-cell share); *dsst2 receives |dSST2| (not in RSD).  Slab reads: RC, rho*Y_k(old)")
-    L.append("// (UC_SLOT_RHOFA+k), SOUR (UC_SLOT_SOUR+q); DT*W_k from P->dtw[k].  Each slab value is")
+    L.append("// (UC_SLOT_RHOFA+k), SOUR (UC_SLOT_SOUR+q); DT*W_k formed as the reference does.  Each slab value is")
     L.append(f"// loaded once and kept in a register while it is among the {SWEEP_CACHE} soonest reused,")
     L.append(f"// else in the lane's {SWEEP_LDS} LDS positions (farthest-next-use replacement at both")
     L.append(f"// levels, planned here); global loads run {SWEEP_PREFETCH} ahead of use.  lds: this lane's")
@@ -477,8 +477,9 @@ def emit_sweep(t, L):
                 c = f"{f32(term['coef'])} * " if term.get("coef") else ""
                 lt.append(f"{c}rc.ld(UC_SLOT_SOUR + {term['sour'] - 1})")
         chunked(f"loss_{k}", lt)
-        S.append(f"const double rhs1_{k} = rc.ld(UC_SLOT_RHOFA + {k - 1}) + P->dtw[{k - 1}] * (rhs1a + rhs1b_{k});"
-                 f" const double rhs2_{k} = -P->dtw[{k - 1}] * loss_{k};"
+        S.append(f"const double dtw_{k} = dt * P->wm[{k - 1}];"
+                 f" const double rhs1_{k} = rc.ld(UC_SLOT_RHOFA + {k - 1}) + dtw_{k} * (rhs1a + rhs1b_{k});"
+                 f" const double rhs2_{k} = -dtw_{k} * loss_{k};"
                  f" const double fnew_{k} = sor2 * s[{k - 1}] + rhs1_{k} * sor1 / (rho - rhs2_{k});"
                  f" rsd += fabs(s[{k - 1}] - fnew_{k}); s[{k - 1}] = fnew_{k} < 0.0 ? 0.0 : fnew_{k};"
                  " asm volatile(\"\" : \"+v\"(rsd));")
