@@ -109,6 +109,7 @@ typedef struct {
     int32_t lpt;        /* 1: tiles of 64 cells run stiffest first (longest-processing-time) */
     int32_t persistent; /* 1: the sweep kernel runs persistent waves over the tile queue */
     int32_t block;      /* block size of the one-thread-per-cell kernels: 64, 128 or 256 */
+    int32_t graph;      /* 1: each sweep segment is replayed from a captured hipGraph */
 } uc_tuning;
 
 typedef struct uc_context uc_context;

@@ -37,14 +37,20 @@ def make(spec):
     return flame_field(li, lj, fuel, seed, dt_seconds=dt, preignition_band=band), mode
 
 
-def run_batch(fields, modes, nsteps, tuning=None):
+def run_batch(fields, modes, nsteps, tuning=None, every_step=False):
     from unicorn_amd.chem import ChemStep, Controls, DeviceField
     devs = [DeviceField(f) for f in fields]
     ctx = ChemStep(sum(f.n for f in fields), max_grids=len(fields))
     if tuning:
         ctx.set_tuning(**tuning)
     ctls = [Controls(dt=f.dt, mode=m) for f, m in zip(fields, modes)]
-    stats = [ctx.step_batch(devs, ctls) for _ in range(nsteps)]
+    stats, states = [], []
+    for _ in range(nsteps):
+        stats.append(ctx.step_batch(devs, ctls))
+        if every_step:
+            states.append([d.to_host() for d in devs])
+    if every_step:
+        return states, stats, devs
     return [d.to_host() for d in devs], stats, devs
 
 
@@ -83,18 +89,23 @@ def test_batch_equals_grids_alone_and_matches_oracle(torch_cuda):
         assert_state_close(got[g], e, f.interior(), f"batch grid {g}")
 
 
-@pytest.mark.parametrize("tuning", [dict(bins=1, lpt=0, persistent=0, block=256),
-                                    dict(bins=16, lpt=1, persistent=1, block=64),
-                                    dict(bins=32, lpt=0, persistent=1, block=128)])
+@pytest.mark.parametrize("tuning", [dict(bins=1, lpt=0, persistent=0, block=256, graph=1),
+                                    dict(bins=16, lpt=1, persistent=1, block=64, graph=1),
+                                    dict(bins=32, lpt=0, persistent=1, block=128, graph=1)])
 def test_tuning_changes_no_result(torch_cuda, tuning):
+    """Every step (each replays the captured sweep-segment graph one or more times, with
+    the step's other launches and host synchronisations in between) against the same
+    steps launched directly without graphs."""
     made = [make(s) for s in GRIDS[:4]]
     fields = [f for f, _ in made]
     modes = [m for _, m in made]
-    base, bst, _ = run_batch([f.copy() for f in fields], modes, 3)
-    got, gst, _ = run_batch([f.copy() for f in fields], modes, 3, tuning)
-    for g in range(len(fields)):
-        assert [s[g].isor1 for s in bst] == [s[g].isor1 for s in gst]
-        assert same(base[g], got[g]), (tuning, g)
+    eager = dict(bins=1, lpt=0, persistent=1, block=256, graph=0)
+    base, bst, _ = run_batch([f.copy() for f in fields], modes, 4, eager, every_step=True)
+    got, gst, _ = run_batch([f.copy() for f in fields], modes, 4, tuning, every_step=True)
+    for step in range(4):
+        for g in range(len(fields)):
+            assert bst[step][g].isor1 == gst[step][g].isor1 and bst[step][g].isor2 == gst[step][g].isor2
+            assert same(base[step][g], got[step][g]), (tuning, step, g)
 
 
 def test_autotune_leaves_state_and_picks_a_valid_tuning(torch_cuda):
@@ -106,6 +117,7 @@ def test_autotune_leaves_state_and_picks_a_valid_tuning(torch_cuda):
     ctx = ChemStep(sum(f.n for f, _ in made), max_grids=len(made))
     t = ctx.autotune(devs, ctls)
     assert 1 <= t["bins"] <= 32 and t["block"] in (64, 128, 256) and t["lpt"] in (0, 1) and t["persistent"] in (0, 1)
+    assert t["graph"] in (0, 1)
     assert ctx.tuning() == t
     for b, d in zip(before, devs):
         assert same(b, d.to_host())

@@ -752,6 +752,15 @@ struct Tuning {
   int32_t lpt = 0;         // tiles stiffest first
   int32_t persistent = 1;  // k_sweep as persistent waves over the tile queue
   int32_t block = 256;     // block size of the one-thread-per-cell kernels
+  int32_t graph = 1;       // sweep segments replayed from a captured hipGraph
+};
+
+// the sweep segment (grid-table upload, queue reset, k_sweep, per-grid row sums and
+// their copy to the host) captured once per layout and tuning, replayed per segment
+struct SegGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<const void*> key;
 };
 }  // namespace
 
@@ -802,6 +811,8 @@ struct uc_context {
   std::vector<int32_t> last_nstar;  // per grid: previous step's species sweep count
   Layout lay;
   Tuning tune;
+  SegGraph seg;
+  hipStream_t gs = nullptr;  // the context's stream for captured sweep segments
   StepRun run;
 };
 
@@ -894,7 +905,8 @@ int uc_create_batch(const uc_scales* scales, int64_t max_cells, int32_t max_grid
     HIPCHK(hipFuncSetAttribute((const void*)k_sweep<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                UC_SWEEP_LDS * 512));
   }
-  for (auto& e : ctx->ev) HIPCHK(hipEventCreate(&e));
+  for (auto& e : ctx->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDefault));
+  HIPCHK(hipStreamCreateWithFlags(&ctx->gs, hipStreamNonBlocking));  // default priority
   *out = ctx.release();
   return UC_OK;
 }
@@ -915,9 +927,13 @@ int uc_destroy(uc_context* ctx) {
                   (void*)ctx->pk.rho, (void*)ctx->pk.y, (void*)ctx->pk.tk, (void*)ctx->pk.ht, (void*)ctx->pk.stmf,
                   (void*)ctx->pk.stnd, (void*)ctx->pk.qdot, (void*)ctx->pk.qdsp, (void*)ctx->pk_iskip})
     if (p) (void)hipFree(p);
+  if (ctx->gs) (void)hipStreamSynchronize(ctx->gs);
+  if (ctx->seg.exec) (void)hipGraphExecDestroy(ctx->seg.exec);
+  if (ctx->seg.graph) (void)hipGraphDestroy(ctx->seg.graph);
   if (ctx->gt) (void)hipHostFree(ctx->gt);
   if (ctx->rows_h) (void)hipHostFree(ctx->rows_h);
   for (auto& e : ctx->ev) (void)hipEventDestroy(e);
+  if (ctx->gs) (void)hipStreamDestroy(ctx->gs);
   delete ctx;
   return UC_OK;
 }
@@ -944,13 +960,13 @@ int uc_set_tuning(uc_context* ctx, const uc_tuning* t) {
   if (!ctx || !t) return fail(UC_E_ARG, "uc_set_tuning: null argument");
   if (t->bins < 1 || t->bins > 32 || (t->block != 64 && t->block != 128 && t->block != 256))
     return fail(UC_E_ARG, "uc_set_tuning: bins must be 1..32 and block 64, 128 or 256");
-  ctx->tune = Tuning{t->bins, t->lpt ? 1 : 0, t->persistent ? 1 : 0, t->block};
+  ctx->tune = Tuning{t->bins, t->lpt ? 1 : 0, t->persistent ? 1 : 0, t->block, t->graph ? 1 : 0};
   return UC_OK;
 }
 
 int uc_get_tuning(const uc_context* ctx, uc_tuning* t) {
   if (!ctx || !t) return fail(UC_E_ARG, "uc_get_tuning: null argument");
-  *t = uc_tuning{ctx->tune.bins, ctx->tune.lpt, ctx->tune.persistent, ctx->tune.block};
+  *t = uc_tuning{ctx->tune.bins, ctx->tune.lpt, ctx->tune.persistent, ctx->tune.block, ctx->tune.graph};
   return UC_OK;
 }
 
