@@ -49,6 +49,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--li", type=int, default=1001)
     p.add_argument("--lj", type=int, default=401)
+    p.add_argument("--workload", choices=("field", "batch"), default="field",
+                   help="field: the 1001x401 C7H16 field (the headline line); batch: 256 grids "
+                        "of uneven size, fuel and dt in one batched step (autotuned first)")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB (5 times): the reference "
                         "kernel for scaling FETCH_SIZE / WRITE_SIZE counter readings")
@@ -110,8 +113,51 @@ def cpu_baseline(field0, nstars, nhs, ncells, steps):
                       f"({steps} steps) subtracted from the slowest process's {max(times):.2f} s"}
 
 
+def batch_bench(args):
+    """256 grids (unicorn_amd.fields.batch_grids) advanced together by uc_step_batch."""
+    import torch
+
+    from unicorn_amd.chem import ChemStep, Controls, DeviceField
+    from unicorn_amd.fields import batch_grids
+
+    grids = batch_grids(256, seed=2)
+    ncell = sum((f.li - 2) * (f.lj - 2) for f in grids)
+    devs = [DeviceField(f) for f in grids]
+    ctls = [Controls(dt=f.dt) for f in grids]
+    ctx = ChemStep(sum(f.n for f in grids), max_grids=len(grids))
+    t0 = time.perf_counter()
+    tuning = ctx.autotune(devs, ctls)
+    t_tune = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        ctx.step_batch(devs, ctls)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = [ctx.step_batch(devs, ctls) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    sweeps = sum(s.cell_sweeps for st in stats for s in st)
+    ms_sweep = sum(st[0].ms_sweeps for st in stats)
+    launches = sum(st[0].sweep_launches for st in stats)
+    out = {"metric": "cell-chem updates/s (batched grids)", "value": ncell * args.steps / el,
+           "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic (unicorn_amd.fields.batch_grids, seed 2)",
+           "config": {"workload": "256 grids, LI log-uniform 31..301, LJ log-uniform 21..151, fuel per grid "
+                                  "CH4/C2H4/C7H16, dt per grid log-uniform 1e-6..1e-5 s, global stops per grid",
+                      "grids": len(grids), "cells": ncell},
+           "tuning": tuning, "autotune_s": t_tune,
+           "k_sweep": {"avg_launch_ms": ms_sweep / launches, "launches": launches,
+                       "achieved_tflops": F_SWEEP * sweeps / (ms_sweep * 1e-3) / 1e12},
+           "sweeps_per_cell_step": sweeps / (ncell * args.steps),
+           "species_sweeps_per_grid_last_step": sorted({s.isor1 for s in stats[-1]})}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
+    if args.workload == "batch":
+        batch_bench(args)
+        return
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return

@@ -183,3 +183,19 @@ def config0(seed: int = 0) -> ChemField:
 def config1(seed: int = 1) -> ChemField:
     """1001 x 401 C7H16 field, 10 % pre-ignition band at 700-1100 K, dt = 1e-5 s."""
     return flame_field(1001, 401, "C7H16", seed, preignition_band=0.10)
+
+
+def batch_grids(ngrids: int = 256, seed: int = 2) -> list[ChemField]:
+    """A batch of independent grids: LI log-uniform in [31, 301], LJ log-uniform in
+    [21, 151] (about 2.1 M cells for 256 grids), fuel per grid drawn from CH4, C2H4
+    and C7H16, dt per grid log-uniform in [1e-6, 1e-5] s; each state built as the
+    61 x 41 field's (flame_field, its own seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for g in range(ngrids):
+        li = int(round(np.exp(rng.uniform(np.log(31), np.log(301)))))
+        lj = int(round(np.exp(rng.uniform(np.log(21), np.log(151)))))
+        fuel = ("CH4", "C2H4", "C7H16")[int(rng.integers(3))]
+        dt = float(np.exp(rng.uniform(np.log(1e-6), np.log(1e-5))))
+        out.append(flame_field(li, lj, fuel, seed=1000 + g, dt_seconds=dt))
+    return out
