@@ -139,7 +139,7 @@ int uc_destroy(uc_context *ctx);
  * and the ideal-gas density update main:1056-1073.  Synchronises the stream
  * (the global stops are decided on the host).  The list of active cells is
  * cached per (ISKIP pointer, LI, LJ): a caller that changes ISKIP in place
- * passes a new array (or a new context). */
+ * calls uc_cells_changed. */
 int uc_step(uc_context *ctx, const uc_field *field, const uc_controls *ctl, void *stream,
             uc_stats *stats);
 
@@ -159,6 +159,10 @@ int uc_step_batch(uc_context *ctx, const uc_field *fields, const uc_controls *ct
 int uc_autotune(uc_context *ctx, const uc_field *fields, const uc_controls *ctls,
                 int32_t ngrids, void *stream, uc_tuning *chosen);
 int uc_set_tuning(uc_context *ctx, const uc_tuning *tuning);
+
+/* The ISKIP contents of the grids changed in place: rebuild the cell list at the
+ * next step. */
+int uc_cells_changed(uc_context *ctx);
 int uc_get_tuning(const uc_context *ctx, uc_tuning *tuning);
 
 /* ---- Fortran drop-in (libunicorn_fshim.so) ------------------------------ */

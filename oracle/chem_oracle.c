@@ -364,6 +364,19 @@ void uco_scales(const double *ref, double *out) {
     memcpy(out + 72, S.hfo, sizeof S.hfo);
 }
 
+/* T from the sensible enthalpy FPZ and the mass fractions y[52] by SOLVEH's Newton
+ * (solveh.f:1170-1230), starting from tk; returns the new tk (nondimensional). */
+double uco_temperature(const double *ref, double fpz, const double *y, double tk) {
+    uc_scales S;
+    uc_scales_init(&S, ref[0], ref[1], ref[2], ref[3], ref[4], ref[5], 1);
+    uc_scales_hfo(&S, MT_NASA);
+    double sq[54];
+    for (int k = 0; k < NSP; ++k) sq[k] = y[k];
+    sq[52] = sq[53] = 0.0;
+    sh_temperature(&S, fpz, sq, &tk);
+    return tk;
+}
+
 /* One chemistry step on a field (see oracle/ref_driver.c for the sequence).
  * rc_dump (optional, ndump x (544+6)) receives RC and SOUR of dump_cells after CKRATES.
  * rsd_hist (optional, isor1 doubles) receives RSD/SPNORM after each species sweep.

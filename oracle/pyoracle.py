@@ -42,6 +42,8 @@ def lib():
         _lib.uco_step.restype = C.c_int
         _lib.uco_rates_cell.restype = None
         _lib.uco_scales.restype = None
+        _lib.uco_temperature.restype = C.c_double
+        _lib.uco_temperature.argtypes = [C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double), C.c_double]
     return _lib
 
 
@@ -108,6 +110,14 @@ def rates_cell(tk, rho, fa, stmf=0.0, stnd=0.0, ref=None):
     fa = np.ascontiguousarray(fa, dtype=np.float64)
     lib().uco_rates_cell(ptr(r), ptr(scal), ptr(fa), ptr(rc), ptr(sour))
     return rc, sour
+
+
+def temperature(fpz, y, tk, ref=None):
+    """T (nondimensional) from the sensible enthalpy FPZ and the 52 mass fractions by
+    SOLVEH's Newton (solveh.f:1170-1230), started from tk."""
+    r = np.array(ref or default_ref(), dtype=np.float64)
+    yy = np.ascontiguousarray(y, dtype=np.float64)
+    return lib().uco_temperature(ptr(r), float(fpz), ptr(yy), float(tk))
 
 
 def scales(ref=None):

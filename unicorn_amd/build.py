@@ -50,8 +50,20 @@ def build_fshim(li: int = 711, lj: int = 131) -> Path:
     return out
 
 
+def build_surrogate() -> Path:
+    """The surrogate's kernels (MLP forward/backward/Adam on BF16 MFMA, features, gate)."""
+    LIBDIR.mkdir(exist_ok=True)
+    out = LIBDIR / "libunicorn_surrogate.so"
+    tmp = LIBDIR / "libunicorn_surrogate.so.tmp"
+    run([HIPCC, *HIP_FLAGS, CSRC / "surrogate_kernels.hip", "-o", tmp], cwd=CSRC)
+    tmp.replace(out)
+    return out
+
+
 def main():
     build_chem()
+    build_fshim()
+    build_surrogate()
 
 
 if __name__ == "__main__":

@@ -57,7 +57,7 @@ class UcField(C.Structure):
 
 EXPORTS = ("uc_abi_version", "uc_last_error", "uc_scales_from_card", "uc_create", "uc_create_batch",
            "uc_destroy", "uc_step", "uc_step_host", "uc_step_batch", "uc_autotune", "uc_set_tuning",
-           "uc_get_tuning")
+           "uc_get_tuning", "uc_cells_changed")
 
 _lib = None
 
@@ -90,6 +90,7 @@ def library() -> C.CDLL:
                                 C.POINTER(UcTuning)]
     lib.uc_set_tuning.argtypes = [C.c_void_p, C.POINTER(UcTuning)]
     lib.uc_get_tuning.argtypes = [C.c_void_p, C.POINTER(UcTuning)]
+    lib.uc_cells_changed.argtypes = [C.c_void_p]
     lib.uc_destroy.argtypes = [C.c_void_p]
     lib.uc_step.argtypes = [C.c_void_p, C.POINTER(UcField), C.POINTER(UcControls), C.c_void_p,
                             C.POINTER(UcStats)]
@@ -231,6 +232,10 @@ class ChemStep:
     def set_tuning(self, bins=1, lpt=0, persistent=1, block=256, graph=1):
         t = UcTuning(bins, lpt, persistent, block, graph)
         _check(self.lib.uc_set_tuning(self.h, C.byref(t)))
+
+    def cells_changed(self):
+        """The fields' ISKIP arrays changed in place: rebuild the cell list next step."""
+        _check(self.lib.uc_cells_changed(self.h))
 
     def tuning(self) -> dict:
         t = UcTuning()

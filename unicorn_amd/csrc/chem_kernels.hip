@@ -964,6 +964,12 @@ int uc_set_tuning(uc_context* ctx, const uc_tuning* t) {
   return UC_OK;
 }
 
+int uc_cells_changed(uc_context* ctx) {
+  if (!ctx) return fail(UC_E_ARG, "uc_cells_changed: null context");
+  ctx->lay.ngrids = 0;
+  return UC_OK;
+}
+
 int uc_get_tuning(const uc_context* ctx, uc_tuning* t) {
   if (!ctx || !t) return fail(UC_E_ARG, "uc_get_tuning: null argument");
   *t = uc_tuning{ctx->tune.bins, ctx->tune.lpt, ctx->tune.persistent, ctx->tune.block, ctx->tune.graph};

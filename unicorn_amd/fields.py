@@ -143,7 +143,8 @@ def mixing_line(Z: np.ndarray, fuel: str) -> np.ndarray:
 
 
 def flame_field(li: int, lj: int, fuel: str = "CH4", seed: int = 0, dt_seconds: float = 1e-5,
-                preignition_band: float = 0.0, scales: Scales | None = None) -> ChemField:
+                preignition_band: float = 0.0, scales: Scales | None = None,
+                t_peak: float = T_PEAK) -> ChemField:
     sc = scales or Scales()
     rng = np.random.Generator(np.random.PCG64(seed))
     z = np.linspace(0.0, Z_LEN, li)
@@ -151,8 +152,8 @@ def flame_field(li: int, lj: int, fuel: str = "CH4", seed: int = 0, dt_seconds: 
     rj = R0 + SLOPE * z
     Z = np.exp(-(r[None, :] / rj[:, None]) ** 2).ravel()          # (li*lj,) i-major, j fastest
     Z = np.clip(Z * (1.0 + 0.02 * rng.uniform(-1.0, 1.0, Z.size)), 0.0, 1.0)
-    T = np.where(Z <= Z_PEAK, T_COLD + (T_PEAK - T_COLD) * Z / Z_PEAK,
-                 T_PEAK + (T_COLD - T_PEAK) * (Z - Z_PEAK) / (1.0 - Z_PEAK))
+    T = np.where(Z <= Z_PEAK, T_COLD + (t_peak - T_COLD) * Z / Z_PEAK,
+                 t_peak + (T_COLD - t_peak) * (Z - Z_PEAK) / (1.0 - Z_PEAK))
     Y = burke_schumann(Z, fuel)
     if preignition_band > 0.0:
         rows = np.arange(li)
