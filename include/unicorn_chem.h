@@ -163,6 +163,15 @@ int uc_set_tuning(uc_context *ctx, const uc_tuning *tuning);
 /* The ISKIP contents of the grids changed in place: rebuild the cell list at the
  * next step. */
 int uc_cells_changed(uc_context *ctx);
+
+/* main:821-828 on one device field (Y<0 -> 0, Y_N2 = 1 - sum Y_1..51, every cell):
+ * the first phase of uc_step, on its own.  uc_set_clip(ctx, 0) makes the context's
+ * steps start without it, for a caller that has applied it already (the surrogate
+ * step clips before its prediction and runs the FP64 step on the rejected cells
+ * after).  No separate counterpart in the reference: main:821-828 runs in the time
+ * loop before CKRATES. */
+int uc_clip(const uc_field *field, void *stream);
+int uc_set_clip(uc_context *ctx, int32_t on);
 int uc_get_tuning(const uc_context *ctx, uc_tuning *tuning);
 
 /* ---- Fortran drop-in (libunicorn_fshim.so) ------------------------------ */

@@ -57,7 +57,7 @@ class UcField(C.Structure):
 
 EXPORTS = ("uc_abi_version", "uc_last_error", "uc_scales_from_card", "uc_create", "uc_create_batch",
            "uc_destroy", "uc_step", "uc_step_host", "uc_step_batch", "uc_autotune", "uc_set_tuning",
-           "uc_get_tuning", "uc_cells_changed")
+           "uc_get_tuning", "uc_cells_changed", "uc_clip", "uc_set_clip")
 
 _lib = None
 
@@ -91,6 +91,8 @@ def library() -> C.CDLL:
     lib.uc_set_tuning.argtypes = [C.c_void_p, C.POINTER(UcTuning)]
     lib.uc_get_tuning.argtypes = [C.c_void_p, C.POINTER(UcTuning)]
     lib.uc_cells_changed.argtypes = [C.c_void_p]
+    lib.uc_clip.argtypes = [C.POINTER(UcField), C.c_void_p]
+    lib.uc_set_clip.argtypes = [C.c_void_p, C.c_int32]
     lib.uc_destroy.argtypes = [C.c_void_p]
     lib.uc_step.argtypes = [C.c_void_p, C.POINTER(UcField), C.POINTER(UcControls), C.c_void_p,
                             C.POINTER(UcStats)]
@@ -173,6 +175,14 @@ class DeviceField:
                          g(self.stnd), g(self.iskip), self.dt, self.fuel)
 
 
+def clip(dev: "DeviceField"):
+    """main:821-828 on a device field (Y<0 -> 0, N2 by closure, every cell), on the
+    current stream: the first phase of a step, on its own."""
+    import torch
+    f = dev.c()
+    _check(library().uc_clip(C.byref(f), C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+
 class ChemStep:
     """A chemistry-step context on the current GPU (scales + scratch)."""
 
@@ -236,6 +246,10 @@ class ChemStep:
     def cells_changed(self):
         """The fields' ISKIP arrays changed in place: rebuild the cell list next step."""
         _check(self.lib.uc_cells_changed(self.h))
+
+    def set_clip(self, on: bool):
+        """Whether this context's steps start with main:821-828 (default on)."""
+        _check(self.lib.uc_set_clip(self.h, 1 if on else 0))
 
     def tuning(self) -> dict:
         t = UcTuning()

@@ -766,6 +766,7 @@ struct SegGraph {
 
 struct uc_context {
   uc_scales sc{};
+  bool clip = true;        // uc_step* start with main:821-828 (uc_set_clip)
   UcParams hp{};
   UcParams* dp = nullptr;
   int64_t max_cells = 0;   // packed cells over all grids
@@ -967,6 +968,20 @@ int uc_set_tuning(uc_context* ctx, const uc_tuning* t) {
 int uc_cells_changed(uc_context* ctx) {
   if (!ctx) return fail(UC_E_ARG, "uc_cells_changed: null context");
   ctx->lay.ngrids = 0;
+  return UC_OK;
+}
+
+int uc_set_clip(uc_context* ctx, int32_t on) {
+  if (!ctx) return fail(UC_E_ARG, "uc_set_clip: null context");
+  ctx->clip = on != 0;
+  return UC_OK;
+}
+
+int uc_clip(const uc_field* f, void* stream) {
+  if (!f || !f->y || f->li < 3 || f->lj < 3) return fail(UC_E_ARG, "uc_clip: bad field");
+  const int64_t n = (int64_t)f->li * f->lj;
+  k_clip<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(f->y, n);
+  HIPCHK(hipGetLastError());
   return UC_OK;
 }
 

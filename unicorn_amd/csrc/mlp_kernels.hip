@@ -79,9 +79,20 @@ __global__ void k_reduce_splits(const float* __restrict__ part, int splits, int6
 }
 
 // ---------------------------------------------------------------- activations
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+// erf by Abramowitz & Stegun, Handbook of Mathematical Functions, 7.1.28:
+// erf(t) = 1 - (1 + a1 t + ... + a6 t^6)^-16 for t >= 0, |error| <= 3e-7 (about 2e-6 in
+// float arithmetic), odd in t.  Branch-free, so a wave evaluates it without divergence;
+// the activations it feeds are rounded to bf16 (relative 4e-3).
+__device__ __forceinline__ float erf_as(float x) {
+  const float t = fabsf(x);
+  const float p = 1.f + t * (0.0705230784f + t * (0.0422820123f + t * (0.0092705272f +
+                  t * (0.0001520143f + t * (0.0002765672f + t * 0.0000430638f)))));
+  const float p2 = p * p, p4 = p2 * p2, p8 = p4 * p4;
+  return copysignf(1.f - __builtin_amdgcn_rcpf(p8 * p8), x);
+}
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erf_as(x * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad(float x) {
-  return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * __expf(-0.5f * x * x);
+  return 0.5f * (1.f + erf_as(x * 0.70710678118654752f)) + x * 0.39894228040143268f * __expf(-0.5f * x * x);
 }
 
 // Z + bias, optionally GELU; bf16 to H [M][ldh] and H^T [N][ldht] (either may be null),

@@ -1,23 +1,30 @@
 """Learned surrogate of the chemistry step, with a gate and FP64 fallback.
 
-A 54 -> 4 x 256 (GELU) -> 53 MLP predicts one chemistry step's dT and dY_k from
-(T, ln dt, Box-Cox(Y_k)).  Weights and activations are BF16, accumulation FP32.
-Forward, backward and Adam run in this project's own kernels
-(``csrc/mlp_kernels.hip``: MFMA 32x32x16 BF16 GEMMs; ``csrc/surrogate_kernels.hip``:
-features, training pairs, the gate).  torch only allocates device memory and
-shuffles indices.
+A 54 -> 4 x 256 (GELU) -> 53 MLP predicts one chemistry step from (T, ln dt,
+Box-Cox(Y_k)).  Its outputs are dT and, per species, the change of Box-Cox(Y_k)
+(lambda = 0.1), each standardised.  Weights and activations are BF16, accumulation
+FP32.  Forward, backward and Adam run in this project's own kernels:
+- ``csrc/mlp_kernels.hip``: MFMA 32x32x16 BF16 GEMMs;
+- ``csrc/surrogate_kernels.hip``: features, training pairs, the fused inference
+  kernel and the gate.
+torch only allocates device memory and shuffles indices.
 
 The training pairs come from this project's FP64 step itself (``ChemStep``), run
-on seeded fields.  At inference a per-cell gate accepts a prediction only under
-three conditions:
-- every input lies in the training range;
-- the applied change conserves C, H and O to ``tol_el``;
-- |sum Y + STMF - 1| <= ``tol_sum``.
-
-The predicted dY is first corrected, with the minimum weighted norm, to conserve C,
-H and O; clipping a negative mass fraction afterwards breaks that and rejects the
-cell. Rejected cells are advanced by the FP64 step (their ISKIP is left as it is and
-every accepted cell is masked out).  No counterpart in the reference.
+on seeded fields.  At inference the predicted change is corrected over species
+1..49, with the least abundance-weighted norm, so that it conserves C, H and O.  A
+per-cell gate then accepts it only when:
+- every input lies in the training range (and STMF, STND in theirs), every output
+  in the range of the training targets, and the new T in SOLVEH's clamp range;
+- the applied change, after clipping and the N2 closure, conserves C, H and O to
+  ``tol_el``;
+- |sum Y + STMF - 1| <= ``tol_sum``;
+- the chemistry is slow: |dT| <= ``tol_dt`` (K), every |dY_k| <= ``tol_dy`` and every
+  predicted change of Box-Cox(Y_k) <= ``tol_bc`` (relative: it flags the radical
+  build-up before an ignition), so that the network's relative error stays a small
+  absolute one (fast cells go to FP64).
+Rejected cells are advanced by the FP64 step (every accepted cell masked out).
+Accepted cells keep their soot scalars, which the network does not predict.  No
+counterpart in the reference.
 """
 from __future__ import annotations
 
@@ -47,14 +54,15 @@ class SurrogateError(RuntimeError):
 
 class Norm(C.Structure):
     _fields_ = [("xmean", C.c_float * NF), ("xsd", C.c_float * NF), ("tmean", C.c_float * NT),
-                ("tsd", C.c_float * NT)]
+                ("tsd", C.c_float * NT), ("tscale", C.c_float * NT)]
 
 
 class Thermo(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("tstr", "hstr", "gasc", "wmstr", "pback", "beta1")] + [
-        ("rwm", C.c_double * LSP), ("hfo", C.c_double * LSP), ("nasa", (C.c_double * 14) * LSP),
-        ("mel", (C.c_double * LSP) * 3), ("proj", (C.c_double * 3) * 49), ("xlo", C.c_double * NF),
-        ("xhi", C.c_double * NF), ("stmf_max", C.c_double), ("stnd_max", C.c_double)]
+        ("rwm", C.c_double * LSP), ("hfo", C.c_double * LSP), ("nasah", (C.c_double * 14) * LSP),
+        ("mel", (C.c_double * LSP) * 3), ("xlo", C.c_double * NF),
+        ("xhi", C.c_double * NF), ("tlo", C.c_double * NT), ("thi", C.c_double * NT), ("tmin", C.c_double),
+        ("tmax", C.c_double), ("stmf_max", C.c_double), ("stnd_max", C.c_double)]
 
 
 def library() -> C.CDLL:
@@ -78,11 +86,11 @@ def library() -> C.CDLL:
         "um_cast_bf16": [v, i64, v, v],
         "um_pack_infer": [v, i, i, i, v, v],
         "us_collect": [v, v, v, v, i64, v, C.c_int32, d, d, v, v, i64, v],
-        "us_colstats": [v, i64, i, i, i64, v, v],
+        "us_colstats": [v, i64, i, i, v, i64, v, v],
         "us_batch": [v, v, v, i, v, v, v, v, v],
         "us_features": [v, v, i64, v, C.c_int32, C.c_int32, d, d, v, v, v],
-        "us_gate": [v, v, v, v, v, v, v, v, v, i64, v, C.c_int32, d, d, d, v, v, v],
-        "us_infer": [v, v, v, v, v, v, v, v, v, v, v, v, i64, v, C.c_int32, d, d, d, v, v],
+        "us_gate": [v, v, v, v, v, v, v, v, i64, v, C.c_int32, d, d, d, d, d, d, v, v, v],
+        "us_infer": [v, v, v, v, v, v, v, v, v, v, v, v, i64, v, C.c_int32, d, d, d, d, d, d, v, v],
         "us_density": [v, v, v, v, i64, i64, v],
         "us_sizes": [v],
     }
@@ -250,8 +258,10 @@ class MLP:
 @dataclass
 class Dataset:
     xraw: "torch.Tensor"   # [P, 54] fp32: T (K), ln dt (s), Y
-    traw: "torch.Tensor"   # [P, 53] fp32: dT (K), dY
+    traw: "torch.Tensor"   # [P, 53] fp32: dT (K), change of Box-Cox(Y)
     rows: int
+    stmf_max: float = 0.0  # largest soot mass fraction / number density among the states
+    stnd_max: float = 0.0
 
 
 def training_fields(seed: int) -> list:
@@ -277,12 +287,14 @@ def training_fields(seed: int) -> list:
 
 
 def generate_pairs(npairs: int, seed: int = 7, steps_per_field: int = 200, rounds: int = 4,
-                   log=print) -> Dataset:
+                   early_steps: int = 10, early_weight: float = 5.0, log=print) -> Dataset:
     """(state, increment) pairs from chained FP64 steps of the training fields.  Each
     round advances 24 seeded fields by steps_per_field steps and, at every step, takes a
-    seeded random subset of each field's interior cells, sized so that ``rounds`` rounds
-    fill npairs rows; the states therefore cover whole trajectories (early ignition to
-    late relaxation), not only their first steps."""
+    seeded random subset of each field's active cells, sized so that ``rounds`` rounds
+    fill npairs rows; the states therefore cover whole trajectories, from the seeded
+    state's first relaxation to late relaxation.  The first ``early_steps`` steps, where
+    the seeded minor species relax by orders of magnitude, are sampled
+    ``early_weight`` times as densely."""
     import torch
 
     from .chem import ChemStep, Controls, DeviceField
@@ -291,7 +303,9 @@ def generate_pairs(npairs: int, seed: int = 7, steps_per_field: int = 200, round
     xraw = torch.empty((npairs, NF), dtype=torch.float32, device="cuda")
     traw = torch.empty((npairs, NT), dtype=torch.float32, device="cuda")
     gen = torch.Generator(device="cuda").manual_seed(seed)
+    weights = [early_weight if s < early_steps else 1.0 for s in range(steps_per_field)]
     rows, rnd = 0, 0
+    smax = torch.zeros(2, dtype=torch.float64, device="cuda")
     while rows < npairs:
         fields = training_fields(seed + rnd)
         devs = [DeviceField(f) for f in fields]
@@ -299,13 +313,17 @@ def generate_pairs(npairs: int, seed: int = 7, steps_per_field: int = 200, round
         ctx = ChemStep(sum(f.n for f in fields), max_grids=len(fields))
         cells = [torch.from_numpy(f.active().astype(np.int32)).cuda() for f in fields]
         nint = sum(int(c.numel()) for c in cells)
-        frac = min(1.0, npairs / (rounds * steps_per_field * nint))
+        frac = npairs / (rounds * sum(weights) * nint)
         logdt = [math.log(f.dt * sc.tmstr) for f in fields]
         for step in range(steps_per_field):
             before = [(d.y.clone(), d.tk.clone()) for d in devs]
+            for d, cl in zip(devs, cells):
+                smax[0] = torch.maximum(smax[0], d.stmf[cl.long()].max())
+                smax[1] = torch.maximum(smax[1], d.stnd[cl.long()].max())
             ctx.step_batch(devs, ctls)
             for d, (y0, t0), cl, ld in zip(devs, before, cells, logdt):
-                take = min(max(1, int(math.ceil(frac * cl.numel()))), npairs - rows)
+                want = int(math.ceil(min(1.0, frac * weights[step]) * cl.numel()))
+                take = min(max(1, want), npairs - rows)
                 if take <= 0:
                     break
                 sel = cl[torch.randperm(cl.numel(), device="cuda", generator=gen)[:take]].contiguous()
@@ -318,25 +336,40 @@ def generate_pairs(npairs: int, seed: int = 7, steps_per_field: int = 200, round
         ctx.close()
         rnd += 1
     torch.cuda.synchronize()
-    return Dataset(xraw, traw, rows)
+    sm = smax.cpu().numpy()
+    return Dataset(xraw, traw, rows, float(sm[0]), float(sm[1]))
+
+
+def target_scales(ds: Dataset, sample: int = 1 << 18, seed: int = 0) -> np.ndarray:
+    """Per-target compression scale: the median |change| over a seeded sample of rows
+    (floored at 1e-6 of the largest |change| in the sample, and at 1e-30)."""
+    import torch
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    idx = torch.randint(0, ds.rows, (min(sample, ds.rows),), device="cuda", generator=gen)
+    a = ds.traw[idx].abs().double().cpu().numpy()
+    return np.maximum(np.maximum(np.median(a, axis=0), 1e-6 * a.max(axis=0)), 1e-30)
 
 
 def column_stats(ds: Dataset) -> dict:
-    """mean, sd, min, max of the transformed features and of the targets."""
+    """mean, sd, min, max of the transformed features and of the compressed targets
+    asinh(d / scale), and the targets' scales."""
     import torch
     lib = library()
     out = {}
+    scale = target_scales(ds)
+    sc_dev = torch.from_numpy(scale.astype(np.float32)).cuda()
     for name, raw, ncol, kind in (("x", ds.xraw, NF, 0), ("t", ds.traw, NT, 1)):
         chunk = 1 << 16
         nchunk = (ds.rows + chunk - 1) // chunk
         part = torch.empty(nchunk * ncol * 4, dtype=torch.float64, device="cuda")
-        _ok(lib.us_colstats(_p(raw), ds.rows, ncol, kind, chunk, _p(part), _stream()))
+        _ok(lib.us_colstats(_p(raw), ds.rows, ncol, kind, _p(sc_dev) if kind else None, chunk, _p(part), _stream()))
         p = part.view(nchunk, ncol, 4).cpu().numpy()
         s1, s2 = p[:, :, 0].sum(0), p[:, :, 1].sum(0)
         mean = s1 / ds.rows
         sd = np.sqrt(np.maximum(s2 / ds.rows - mean * mean, 0.0))
         sd = np.where(sd > 1e-12 * np.maximum(1.0, np.abs(mean)), sd, 1.0)
         out[name] = dict(mean=mean, sd=sd, lo=p[:, :, 2].min(0), hi=p[:, :, 3].max(0))
+    out["t"]["scale"] = scale.astype(np.float32).astype(np.float64)
     return out
 
 
@@ -346,6 +379,7 @@ def norm_block(stats: dict) -> Norm:
         nm.xmean[i], nm.xsd[i] = stats["x"]["mean"][i], stats["x"]["sd"][i]
     for i in range(NT):
         nm.tmean[i], nm.tsd[i] = stats["t"]["mean"][i], stats["t"]["sd"][i]
+        nm.tscale[i] = stats["t"]["scale"][i]
     return nm
 
 
@@ -435,25 +469,24 @@ def thermo_block(stats: dict, sc: Scales | None = None) -> Thermo:
     for k in range(LSP):
         th.rwm[k] = 1.0 / sc.wm[k]
         th.hfo[k] = sc.hfo[k]
-        for i in range(14):
-            th.nasa[k][i] = pol[k, i]
+        for r in (0, 7):  # per range: a_i / i for i = 1..5, a_6, a_7 (unused)
+            for i in range(7):
+                th.nasah[k][r + i] = pol[k, r + i] / (i + 1) if i < 5 else pol[k, r + i]
         for e in range(3):
             th.mel[e][k] = mel[e, k]
-    # minimum S-weighted correction over species 1..49 (AR, HE, N2 excluded: N2 closes)
-    A = mel[:, :49]
-    S = np.diag(np.asarray(stats["t"]["sd"][1:50], np.float64) ** 2)
-    G = S @ A.T @ np.linalg.inv(A @ S @ A.T)
-    for k in range(49):
-        for e in range(3):
-            th.proj[k][e] = G[k, e]
     # the hull, widened by HULL_MARGIN of each feature's range (the stored bounds are the
     # float32 features' extremes; a state exactly on the edge must pass)
     for i in range(NF):
         lo, hi = float(stats["x"]["lo"][i]), float(stats["x"]["hi"][i])
         m = HULL_MARGIN * (hi - lo) + 1e-6 * max(1.0, abs(lo), abs(hi))
         th.xlo[i], th.xhi[i] = lo - m, hi + m
-    th.stmf_max = float(stats.get("stmf_max", 0.0))
-    th.stnd_max = float(stats.get("stnd_max", 0.0))
+    for i in range(NT):
+        lo, hi = float(stats["t"]["lo"][i]), float(stats["t"]["hi"][i])
+        m = HULL_MARGIN * (hi - lo)
+        th.tlo[i], th.thi[i] = lo - m, hi + m
+    th.tmin, th.tmax = sc.tstr - 1.0, 5000.0  # solveh.f:38,1229-1230
+    th.stmf_max = float(stats.get("stmf_max", 0.0)) * (1.0 + HULL_MARGIN)
+    th.stnd_max = float(stats.get("stnd_max", 0.0)) * (1.0 + HULL_MARGIN)
     return th
 
 
@@ -465,7 +498,7 @@ class SurrogateStep:
     which the tests use to cross-check the fused kernel."""
 
     def __init__(self, net: MLP, stats: dict, field, tol_el: float = 1e-6, tol_sum: float = 1e-6,
-                 fused: bool = True):
+                 tol_dt: float = 5.0, tol_dy: float = 2e-3, tol_bc: float = 0.1, fused: bool = True):
         import torch
 
         from .chem import ChemStep
@@ -474,7 +507,7 @@ class SurrogateStep:
         self.sc = Scales()
         self.nm = _to_device(norm_block(stats))
         self.th = _to_device(thermo_block(stats, self.sc))
-        self.tol_el, self.tol_sum = tol_el, tol_sum
+        self.tol_el, self.tol_sum, self.tol_dt, self.tol_dy, self.tol_bc = tol_el, tol_sum, tol_dt, tol_dy, tol_bc
         self.fused = fused
         self.li, self.lj = field.li, field.lj
         self.cells = torch.from_numpy(field.active().astype(np.int32)).cuda()  # ISKIP < 3 (ckrates.f:93)
@@ -488,6 +521,7 @@ class SurrogateStep:
             self.X = torch.zeros((self.bpad, NPAD), dtype=torch.int16, device="cuda")
             self.reject = torch.zeros(field.n, dtype=torch.int32, device="cuda")
         self.fp64 = ChemStep(field.n)
+        self.fp64.set_clip(False)  # step() clips every cell before the prediction
 
     def predict(self, dev, ctl, out=None):
         """The network and the gate only (no fallback): accepted cells of dev get their new
@@ -503,34 +537,40 @@ class SurrogateStep:
             _ok(self.lib.us_infer(self.wptr, self.bptr, _p(self.nm), _p(self.th), _p(dev.y), _p(dev.tk), _p(dev.ht),
                                   _p(dev.stmf), _p(dev.stnd), _p(dev.iskip), _p(self.mask),
                                   None if out is None else _p(out), n, _p(self.cells), self.ncells, log_dt,
-                                  self.tol_el, self.tol_sum, _p(self.stats), s))
+                                  self.tol_el, self.tol_sum, self.tol_dt, self.tol_dy, self.tol_bc, _p(self.stats),
+                                  s))
         else:
             _ok(self.lib.us_features(_p(dev.y), _p(dev.tk), n, _p(self.cells), self.ncells, self.bpad,
                                      self.sc.tstr, log_dt, _p(self.nm), _p(self.X), s))
             o = self.net.forward(self.X, self.bpad)
             if out is not None:
                 out.copy_(o[: self.ncells])
-            _ok(self.lib.us_gate(_p(o), _p(self.nm), _p(self.th), _p(dev.y), _p(dev.tk), _p(dev.ht), _p(dev.rho),
+            _ok(self.lib.us_gate(_p(o), _p(self.nm), _p(self.th), _p(dev.y), _p(dev.tk), _p(dev.ht),
                                  _p(dev.stmf), _p(dev.stnd), n, _p(self.cells), self.ncells, log_dt,
-                                 self.tol_el, self.tol_sum, _p(self.reject), _p(self.stats), s))
+                                 self.tol_el, self.tol_sum, self.tol_dt, self.tol_dy, self.tol_bc, _p(self.reject),
+                                 _p(self.stats), s))
             self.mask.copy_(dev.iskip)
             acc = self.cells[self.reject[self.cells.long()] == 0].long()
             self.mask[acc] = 10
         return self.stats
 
     def step(self, dev, ctl) -> dict:
-        """Advance the device field by one step: surrogate where the gate accepts, the
-        FP64 step on the rejected cells (every accepted cell masked out).  Returns counts
-        and timings."""
+        """Advance the device field by one step as the FP64 step would: main:821-828 on
+        every cell, then the surrogate where the gate accepts and the FP64 step (without
+        its own clip) on the rejected cells, every accepted cell masked out.  Returns
+        counts and timings."""
         import torch
+
+        from .chem import clip
         t0 = torch.cuda.Event(enable_timing=True)
         t1 = torch.cuda.Event(enable_timing=True)
         t2 = torch.cuda.Event(enable_timing=True)
         t0.record()
+        clip(dev)
         self.predict(dev, ctl)
         t1.record()
         st = self.stats.cpu().numpy()
-        nrej = int(st[1] + st[2] + st[3])
+        nrej = int(st[1:5].sum())
         fp = None
         if nrej:
             saved = dev.iskip
@@ -544,9 +584,9 @@ class SurrogateStep:
             _ok(self.lib.us_density(_p(self.th), _p(dev.y), _p(dev.tk), _p(dev.rho), self.li, self.lj, _stream()))
         t2.record()
         torch.cuda.synchronize()
-        raw = st[4:7].astype(np.int64).view(np.float64)
+        raw = st[5:8].astype(np.int64).view(np.float64)
         return {"accepted": int(st[0]), "rejected": nrej, "out_of_range": int(st[1]), "residual": int(st[2]),
-                "sum": int(st[3]), "raw_element_residual_max": raw.tolist(),
+                "sum": int(st[3]), "active": int(st[4]), "raw_element_residual_max": raw.tolist(),
                 "ms_surrogate": t0.elapsed_time(t1), "ms_total": t0.elapsed_time(t2),
                 "fp64_sweeps": None if fp is None else fp.isor1}
 
@@ -563,6 +603,7 @@ def save(path, net: MLP, stats: dict, meta: dict):
 def load(path) -> tuple[MLP, dict]:
     z = np.load(path, allow_pickle=False)
     stats = {g: {k: z[f"stat_{g}_{k}"] for k in ("mean", "sd", "lo", "hi")} for g in ("x", "t")}
+    stats["t"]["scale"] = z["stat_t_scale"]
     stats["stmf_max"] = float(z["stmf_max"])
     stats["stnd_max"] = float(z["stnd_max"])
     net = MLP(0)

@@ -40,8 +40,8 @@ def cmd_train(a):
     t_gen = time.perf_counter() - t0
     stats = S.column_stats(ds)
     net, info = S.train(ds, stats, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
-    stats["stmf_max"] = 0.0
-    stats["stnd_max"] = 0.0
+    stats["stmf_max"] = ds.stmf_max
+    stats["stnd_max"] = ds.stnd_max
     S.save(a.out, net, stats, {"pairs": ds.rows, "epochs": a.epochs, "batch": a.batch, "lr": a.lr})
     torch.cuda.synchronize()
     print(json.dumps({"train": {"pairs": ds.rows, "generate_s": t_gen, "train_s": info["seconds"],
@@ -84,7 +84,8 @@ def cmd_infer(a):
     log(f"fp64: {ms_fp64:.3f} ms/step")
     # 2. throughput, gated surrogate + FP64 fallback
     dev = DeviceField(f)
-    sstep = SurrogateStep(net, stats, f, tol_el=a.tol_el, tol_sum=a.tol_sum)
+    sstep = SurrogateStep(net, stats, f, tol_el=a.tol_el, tol_sum=a.tol_sum, tol_dt=a.tol_dt, tol_dy=a.tol_dy,
+                          tol_bc=a.tol_bc)
     ms_sur, per = timed(lambda i: sstep.step(dev, ctl))
     rej = [p["rejected"] / nact for p in per]
     log(f"surrogate: {ms_sur:.3f} ms/step, fallback {np.mean(rej):.4f}")
@@ -118,7 +119,8 @@ def cmd_infer(a):
         "sumY_residual_fp64": float(sum_ref), "sumY_residual_surrogate": float(sum_sur),
         "ms_per_step_fp64": ms_fp64, "ms_per_step_surrogate": ms_sur,
         "speedup": ms_fp64 / ms_sur,
-        "gate_rejections_last_step": {k: per[-1][k] for k in ("out_of_range", "residual", "sum")},
+        "gate": {"tol_el": a.tol_el, "tol_sum": a.tol_sum, "tol_dT_K": a.tol_dt, "tol_dY": a.tol_dy, "tol_dBC": a.tol_bc},
+        "gate_rejections_last_step": {k: per[-1][k] for k in ("out_of_range", "residual", "sum", "active")},
         "raw_element_residual_max_last_step": per[-1]["raw_element_residual_max"],
         "ms_network_and_gate_last_step": per[-1]["ms_surrogate"],
         "one_step_dT_by_step": one_dT[:: max(1, a.steps // 20)],
@@ -140,6 +142,9 @@ def main(argv=None):
     i.add_argument("--steps", type=int, default=200)
     i.add_argument("--tol-el", type=float, default=1e-6)
     i.add_argument("--tol-sum", type=float, default=1e-6)
+    i.add_argument("--tol-dt", type=float, default=5.0)
+    i.add_argument("--tol-dy", type=float, default=2e-3)
+    i.add_argument("--tol-bc", type=float, default=0.1)
     a = p.parse_args(argv)
     {"train": cmd_train, "infer": cmd_infer}[a.cmd](a)
 
