@@ -33,6 +33,8 @@
 #include <vector>
 
 #include <hipcub/device/device_radix_sort.hpp>
+#include <hipcub/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include "../../include/unicorn_chem.h"
 
@@ -795,8 +797,10 @@ struct uc_context {
   int32_t* vals[2]{};
   int32_t* order = nullptr;     // [max_tiles] tile order (the work queue)
   int32_t* queue = nullptr;     // persistent k_sweep: next queue position
-  void* sort_tmp = nullptr;
+  void* sort_tmp = nullptr;     // scratch of the radix sorts and of the cell selection
   size_t sort_tmp_bytes = 0;
+  int32_t* nsel = nullptr;      // selected-cell count (device) and its pinned host copy
+  int32_t* nsel_h = nullptr;
   uint8_t* done = nullptr;      // [max_cells]
   GridTab* gt_dev = nullptr;    // [max_grids]
   GridTab* gt = nullptr;        // pinned host mirror
@@ -890,7 +894,15 @@ int uc_create_batch(const uc_scales* scales, int64_t max_cells, int32_t max_grid
   HIPCHK(hipMalloc(&ctx->queue, sizeof(int32_t) * 64));
   HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, ctx->sort_tmp_bytes, ctx->keys[0], ctx->keys[1], ctx->vals[0],
                                             ctx->vals[1], (int)std::max(max_cells, mt), 0, 32));
+  {
+    size_t sel = 0;
+    HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, sel, rocprim::counting_iterator<int32_t>(0), ctx->done,
+                                         ctx->base, ctx->nsel, (int)max_cells));
+    ctx->sort_tmp_bytes = std::max(ctx->sort_tmp_bytes, sel);
+  }
   HIPCHK(hipMalloc(&ctx->sort_tmp, std::max<size_t>(ctx->sort_tmp_bytes, 256)));
+  HIPCHK(hipMalloc(&ctx->nsel, sizeof(int32_t) * 64));
+  HIPCHK(hipHostMalloc(&ctx->nsel_h, sizeof(int32_t) * 64, hipHostMallocDefault));
   HIPCHK(hipMalloc(&ctx->done, max_cells));
   HIPCHK(hipMalloc(&ctx->gt_dev, sizeof(GridTab) * max_grids));
   HIPCHK(hipHostMalloc(&ctx->gt, sizeof(GridTab) * max_grids, hipHostMallocDefault));
@@ -926,8 +938,10 @@ int uc_destroy(uc_context* ctx) {
                   (void*)ctx->queue, ctx->sort_tmp, (void*)ctx->done, (void*)ctx->gt_dev, (void*)ctx->grid_ptrs,
                   (void*)ctx->rsd_part, (void*)ctx->res_part, (void*)ctx->rows, (void*)ctx->counters,
                   (void*)ctx->pk.rho, (void*)ctx->pk.y, (void*)ctx->pk.tk, (void*)ctx->pk.ht, (void*)ctx->pk.stmf,
-                  (void*)ctx->pk.stnd, (void*)ctx->pk.qdot, (void*)ctx->pk.qdsp, (void*)ctx->pk_iskip})
+                  (void*)ctx->pk.stnd, (void*)ctx->pk.qdot, (void*)ctx->pk.qdsp, (void*)ctx->pk_iskip,
+                  (void*)ctx->nsel})
     if (p) (void)hipFree(p);
+  if (ctx->nsel_h) (void)hipHostFree(ctx->nsel_h);
   if (ctx->gs) (void)hipStreamSynchronize(ctx->gs);
   if (ctx->seg.exec) (void)hipGraphExecDestroy(ctx->seg.exec);
   if (ctx->seg.graph) (void)hipGraphDestroy(ctx->seg.graph);

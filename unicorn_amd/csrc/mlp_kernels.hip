@@ -20,6 +20,7 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint16_t bf16_t;  // bit pattern in memory
 
 thread_local std::string g_err;
@@ -191,16 +192,16 @@ __global__ void k_pack_w(const float* __restrict__ W, int out, int in, int ldw, 
 }
 
 // fp32 W [out][in] -> bf16 [out][in] for the fused inference kernel.  perm = 1 orders
-// each aligned 16-column block as the previous layer's MFMA result fragment hands the
-// activations over: position 8h + j holds input 8(j/4) + 4h + j%4 (quads 1 and 2 swap).
+// each aligned 32-column block as a 16x16x32 MFMA result fragment hands the previous
+// layer's activations over: position 8g + j holds input 16(j/4) + 4g + j%4.
 __global__ void k_pack_infer(const float* __restrict__ W, int out, int in, int perm, bf16_t* __restrict__ Wp) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)out * in) return;
   const int o = (int)(i / in), k = (int)(i % in);
   int src = k;
   if (perm) {
-    const int q = (k % 16) / 4;
-    src = k - (k % 16) + 4 * (q == 1 ? 2 : (q == 2 ? 1 : q)) + k % 4;
+    const int q = k % 32, g = q / 8, j = q % 8;
+    src = k - q + 16 * (j / 4) + 4 * g + j % 4;
   }
   Wp[i] = to_bf16(W[(int64_t)o * in + src]);
 }
@@ -289,7 +290,7 @@ int um_pack_w(const float* W, int out, int in, int ldw, uint16_t* Wb, int ldb, u
 }
 
 int um_pack_infer(const float* W, int out, int in, int perm, uint16_t* Wp, void* stream) {
-  if (in % 16) return fail(-1, "um_pack_infer: in must be a multiple of 16");
+  if (in % 32) return fail(-1, "um_pack_infer: in must be a multiple of 32");
   k_pack_infer<<<blocks((int64_t)out * in, 256), 256, 0, (hipStream_t)stream>>>(W, out, in, perm, (bf16_t*)Wp);
   HIPCHK(hipGetLastError());
   return 0;

@@ -177,19 +177,21 @@ __device__ __forceinline__ double y_after(double P, float d) {
   return u8 * u2;
 }
 
-// The gate's arithmetic for one cell.  D d: d(j), the change of output j (j = 0: dT in K;
-// j = 1 + k: the Box-Cox change of Y_k); P p: p(k) = Y0_k^0.1; Y0 y0f: y0f(k) = Y0_k.
+// The gate of one cell, one thread.  D d: d(j), the change of output j (j = 0: dT in K;
+// j = 1 + k: the Box-Cox change of Y_k), from out_change; PB pb: pb(k) = Y0_k^0.1.
 // flags_ok: the features lie in the hull and every output in range; slow_bc: every
-// species' Box-Cox change within tol_bc.
-template <class D, class P, class Y0>
-__device__ int gate_cell(D d, P p, Y0 y0f, const Thermo* __restrict__ th, double* __restrict__ y, double* __restrict__ tk,
-                         double* __restrict__ ht, double stm, double stn, int64_t n, int64_t c, bool flags_ok,
-                         bool slow_bc, double tol_el, double tol_sum, double tol_dt, double tol_dy, double* rr) {
+// species' Box-Cox change within tol_bc.  Species are summed in index order.
+template <class D, class PB>
+__device__ int gate_cell(D d, PB pb, const Thermo* __restrict__ th, double* __restrict__ y,
+                         double* __restrict__ tk, double* __restrict__ ht, double stm, double stn, int64_t n,
+                         int64_t c, bool flags_ok, bool slow_bc, double tol_el, double tol_sum, double tol_dt,
+                         double tol_dy, double* rr) {
   // raw residual r = A dY and the weighted normal matrix M = A W A^T (symmetric 3x3)
   double m[6] = {0, 0, 0, 0, 0, 0};
   rr[0] = rr[1] = rr[2] = 0.0;
+#pragma unroll 1
   for (int k = 0; k < 49; ++k) {
-    const double y0 = y0f(k), y1 = y_after(p(k), d(1 + k));
+    const double y0 = y[(int64_t)k * n + c], y1 = y_after(pb(k), d(1 + k));
     const double w = fmax(y0, y1), a0 = th->mel[0][k], a1 = th->mel[1][k], a2 = th->mel[2][k];
     rr[0] += a0 * (y1 - y0);
     rr[1] += a1 * (y1 - y0);
@@ -209,13 +211,14 @@ __device__ int gate_cell(D d, P p, Y0 y0f, const Thermo* __restrict__ th, double
   }
   // Y1 of species k < 51: predicted, corrected, clipped (both passes use this)
   auto y1_of = [&](int k, double y0) {
-    double y1 = y_after(p(k), d(1 + k));
+    double y1 = y_after(pb(k), d(1 + k));
     if (k < 49) y1 -= fmax(y0, y1) * (th->mel[0][k] * lam[0] + th->mel[1][k] * lam[1] + th->mel[2][k] * lam[2]);
     return y1 < 0.0 ? 0.0 : y1;
   };
   double tot = 0.0, dm[3] = {0.0, 0.0, 0.0}, m1[3] = {0.0, 0.0, 0.0}, dmax = 0.0;
+#pragma unroll 1
   for (int k = 0; k < 51; ++k) {
-    const double y0 = y0f(k), y1 = y1_of(k, y0);
+    const double y0 = y[(int64_t)k * n + c], y1 = y1_of(k, y0);
     tot += y1;
     dmax = fmax(dmax, fabs(y1 - y0));
     for (int e = 0; e < 3; ++e) {
@@ -226,7 +229,7 @@ __device__ int gate_cell(D d, P p, Y0 y0f, const Thermo* __restrict__ th, double
   const double ftot = tot + stm;
   const bool over = ftot > 1.0;
   const double yn = over ? 0.0 : 1.0 - ftot;
-  const double y0n = y0f(51);
+  const double y0n = y[(int64_t)51 * n + c];
   dmax = fmax(dmax, fabs(yn - y0n));
   double res = 0.0;
   for (int e = 0; e < 3; ++e) {
@@ -242,14 +245,15 @@ __device__ int gate_cell(D d, P p, Y0 y0f, const Thermo* __restrict__ th, double
   if (why) return why;
   const int ip = T > 1000.0 ? 7 : 0;
   double pp[6] = {0, 0, 0, 0, 0, 0}, hf = 0.0;
+#pragma unroll 1
   for (int k = 0; k < NSPS; ++k) {
     double y1 = yn;
     if (k < 51) {
-      y1 = y1_of(k, y0f(k));
+      y1 = y1_of(k, y[(int64_t)k * n + c]);
       if (over) y1 = y1 / ftot;
     }
     const double sw = y1 * th->rwm[k];
-    for (int i = 0; i < 6; ++i) pp[i] += th->nasah[k][ip + i] * sw;
+    for (int e = 0; e < 6; ++e) pp[e] += th->nasah[k][ip + e] * sw;
     hf += y1 * th->hfo[k] * th->hstr;
     y[(int64_t)k * n + c] = y1;
   }
@@ -276,175 +280,178 @@ __device__ __forceinline__ void gate_stats(int why, const double* rr, unsigned l
   }
 }
 
-// the gate as a kernel of its own, on the outputs of the separate-kernel network
-// (out [ncells][64]); reject[c] = 1 where rejected, 0 where accepted
-__global__ void k_gate(const float* __restrict__ out, const Norm* __restrict__ nm, const Thermo* __restrict__ th,
-                       double* __restrict__ y, double* __restrict__ tk, double* __restrict__ ht,
-                       const double* __restrict__ stmf, const double* __restrict__ stnd, int64_t n,
-                       const int32_t* __restrict__ cells, int32_t ncells, double log_dt, double tol_el,
-                       double tol_sum, double tol_dt, double tol_dy, double tol_bc, int32_t* __restrict__ reject,
-                       unsigned long long* __restrict__ stats) {
+// The gate as a kernel, one thread per listed cell, on the network's standardised
+// outputs: output j of list position q at out[q * sq + j * sj] (row-major [q][64] from
+// the separate-kernel network, species-major [64][ldo] from k_network).  pbuf: Y0^0.1
+// as [52][ldo] (k_network writes it; null: computed here, by the same function).
+// reject[c] = 1 where rejected, 0 where accepted (may be null); mask[c] = 10 where
+// accepted, iskip[c] where rejected (may be null): the ISKIP with which the FP64 step
+// then advances only the rejected cells.
+__global__ void __launch_bounds__(256) k_gate(const float* __restrict__ out, int64_t sq, int64_t sj,
+                                              const double* __restrict__ pbuf, int64_t ldo,
+                                              const Norm* __restrict__ nm, const Thermo* __restrict__ th,
+                                              double* __restrict__ y, double* __restrict__ tk,
+                                              double* __restrict__ ht, const double* __restrict__ stmf,
+                                              const double* __restrict__ stnd, int64_t n,
+                                              const int32_t* __restrict__ cells, int32_t ncells, double log_dt,
+                                              double tol_el, double tol_sum, double tol_dt, double tol_dy,
+                                              double tol_bc, int32_t* __restrict__ reject,
+                                              const int32_t* __restrict__ iskip, int32_t* __restrict__ mask,
+                                              unsigned long long* __restrict__ stats) {
+  __shared__ float sd[NT][256];  // the changes of this thread's cell, [output][thread]
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int why = -1;
   double rr[3] = {0.0, 0.0, 0.0};
   if (q < ncells) {
     const int64_t c = cells[q];
+    auto pb = [&](int k) { return pbuf ? pbuf[(int64_t)k * ldo + q] : bc_base(y[(int64_t)k * n + c]); };
     const double T0 = tk[c] * th->tstr;
-    bool ok = true, slow = true;
-    for (int j = 0; j < NF && ok; ++j) {
-      const double v = j == 0 ? T0 : (j == 1 ? log_dt : (bc_base(y[(int64_t)(j - 2) * n + c]) - 1.0) / 0.1);
-      ok = v >= th->xlo[j] && v <= th->xhi[j];
+    bool ok = T0 >= th->xlo[0] && T0 <= th->xhi[0] && log_dt >= th->xlo[1] && log_dt <= th->xhi[1];
+    for (int k = 0; k < NSPS; ++k) {
+      const double v = (pb(k) - 1.0) / 0.1;
+      ok = ok && v >= th->xlo[k + 2] && v <= th->xhi[k + 2];
     }
-    const float* o = out + q * NPAD;
-    float dv[NT];
+    bool slow = true;
     for (int j = 0; j < NT; ++j) {
-      bool a, b;
-      dv[j] = out_change(o[j], j, nm, th, tol_bc, a, b);
-      ok = ok && a;
-      slow = slow && b;
+      bool r0, r1;
+      sd[j][threadIdx.x] = out_change(out[q * sq + j * sj], j, nm, th, tol_bc, r0, r1);
+      ok = ok && r0;
+      slow = slow && r1;
     }
-    why = gate_cell([&](int j) { return dv[j]; }, [&](int k) { return bc_base(y[(int64_t)k * n + c]); },
-                    [&](int k) { return y[(int64_t)k * n + c]; }, th, y, tk, ht, stmf[c], stnd[c], n, c, ok, slow,
+    why = gate_cell([&](int j) { return sd[j][threadIdx.x]; }, pb, th, y, tk, ht, stmf[c], stnd[c], n, c, ok, slow,
                     tol_el, tol_sum, tol_dt, tol_dy, rr);
-    reject[c] = why != 0;
+    if (reject) reject[c] = why != 0;
+    if (mask) mask[c] = why == 0 ? 10 : iskip[c];
   }
   gate_stats(why, rr, stats);
 }
 
-// ---- fused inference: features -> network -> gate, 32 cells per wave, one launch.
-// The network runs transposed, Z^T = W H^T: W is the MFMA's A operand (rows = output
-// neurons) and H^T its B operand (columns = the wave's 32 cells), so a layer's result
-// fragment (lane l: cell l%32, neurons 8(r/4) + 4(l/32) + r%4) is, after bias, GELU
-// and rounding to bf16, the next layer's B operand without leaving registers: k-step
-// 2t+p takes registers 8p..8p+7 of result tile t, and the next layer's weight columns
-// are stored in that order (k_pack_infer, perm = 1).  Only the 64 outputs per cell pass
-// through LDS, to the lane that runs the cell's FP64 gate.
+// ---- the network on a field: features -> MLP -> outputs, 16 cells per wave, one launch
+// (k_gate follows).
+// The network runs transposed, Z^T = W H^T, on v_mfma_f32_16x16x32_bf16: W is the A
+// operand (rows = output neurons) and H^T the B operand (columns = the wave's 16
+// cells).  Lane l = 16g + cl holds cell cl; a layer's result tile t gives it neurons
+// 16t + 4g + r (r = 0..3), which, after bias, GELU and rounding to bf16, are the next
+// layer's B operand without leaving registers: k-step s takes tile 2s (j = 0..3) and
+// tile 2s+1 (j = 4..7), and the next layer's weight columns are stored in that order
+// (k_pack_infer, perm = 1).  The weights pass through LDS (infer_layer), the
+// activations never leave registers; the 64 outputs per cell go to out and the Y^0.1
+// of its features to pbuf, both species-major for the gate's coalesced loads.
 struct InferArgs {
   const bf16_t* w[5];  // packed weights: [256][64], 3 x [256][256] (perm), [64][256] (perm)
   const float* b[5];   // biases: 4 x [256], [64]
   const Norm* nm;
-  const Thermo* th;
-  double *y, *tk, *ht;  // rho is left to the step's finish (FP64 fallback or us_density)
-  const double *stmf, *stnd;
-  const int32_t* iskip;
-  int32_t* mask;       // ISKIP for the FP64 fallback: 10 where accepted, ISKIP[c] where rejected
-  float* out;          // optional [ncells][64] standardised outputs (tests), may be null
+  const double *y, *tk;
+  float* out;          // [64][ldo] standardised outputs, species-major
+  double* pbuf;        // [52][ldo] Y^0.1, for the gate
+  int64_t ldo;
   const int32_t* cells;
   int64_t n;
   int32_t ncells;
-  double log_dt, tol_el, tol_sum, tol_dt, tol_dy, tol_bc;
-  unsigned long long* stats;  // as k_gate
+  double tstr, log_dt;
 };
 
+// One layer, Z^T = W H^T, for the block's 64 cells.  The block stages each k-step's
+// weight slab (NTILES*16 rows x 32 columns, bf16) through LDS, double-buffered, so the
+// four waves read every weight once from L2 between them; each wave then reads its A
+// fragments (16 bytes a lane, conflict-free) from LDS.  Every thread of the block calls
+// this, once per layer, in the same order.
 template <int NTILES, int KSTEPS>
-__device__ __forceinline__ void infer_layer(const bf16_t* __restrict__ w, const bf16x8* in, f32x16* acc, int cl,
-                                            int h) {
-  constexpr int K = KSTEPS * 16;
+__device__ __forceinline__ void infer_layer(const bf16_t* __restrict__ w, const bf16x8* in, f32x4* acc, int cl,
+                                            int g, bf16_t* __restrict__ slab) {
+  constexpr int K = KSTEPS * 32, ROWS = NTILES * 16, CHUNKS = ROWS * 4 / 256;  // 16-byte chunks a thread
+  constexpr int BUF = 256 * 32;                                                   // one buffer, bf16 elements
+  const int tid = threadIdx.x;
 #pragma unroll
   for (int t = 0; t < NTILES; ++t)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
+    for (int r = 0; r < 4; ++r) acc[t][r] = 0.f;
+  bf16x8 ld[CHUNKS];
+#pragma unroll
+  for (int i = 0; i < CHUNKS; ++i) {
+    const int ch = tid + 256 * i, row = ch / 4, col = (ch % 4) * 8;
+    ld[i] = *(const bf16x8*)(w + row * K + col);
+  }
+#pragma unroll
+  for (int i = 0; i < CHUNKS; ++i) *(bf16x8*)(slab + (tid + 256 * i) * 8) = ld[i];
+  __syncthreads();
 #pragma unroll
   for (int s = 0; s < KSTEPS; ++s) {
-    bf16x8 wa[NTILES];
+    const bf16_t* cur = slab + (s % 2) * BUF;
+    if (s + 1 < KSTEPS)
 #pragma unroll
-    for (int t = 0; t < NTILES; ++t) wa[t] = *(const bf16x8*)(w + (t * 32 + cl) * K + 16 * s + 8 * h);
+      for (int i = 0; i < CHUNKS; ++i) {
+        const int ch = tid + 256 * i, row = ch / 4, col = (ch % 4) * 8;
+        ld[i] = *(const bf16x8*)(w + row * K + 32 * (s + 1) + col);
+      }
 #pragma unroll
-    for (int t = 0; t < NTILES; ++t) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[t], in[s], acc[t], 0, 0, 0);
+    for (int t = 0; t < NTILES; ++t) {
+      const bf16x8 wa = *(const bf16x8*)(cur + (t * 16 + cl) * 32 + 8 * g);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, in[s], acc[t], 0, 0, 0);
+    }
+    if (s + 1 < KSTEPS)
+#pragma unroll
+      for (int i = 0; i < CHUNKS; ++i) *(bf16x8*)(slab + ((s + 1) % 2) * BUF + (tid + 256 * i) * 8) = ld[i];
+    __syncthreads();
   }
 }
 
-__device__ __forceinline__ void infer_activate(const f32x16* acc, const float* __restrict__ b, int h, bf16x8* act) {
+__device__ __forceinline__ void infer_activate(const f32x4* acc, const float* __restrict__ b, int g, bf16x8* act) {
 #pragma unroll
-  for (int t = 0; t < 8; ++t)
+  for (int s = 0; s < 8; ++s)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) act[2 * t + r / 8][r % 8] = (__bf16)gelu(acc[t][r] + b[32 * t + 8 * (r / 4) + 4 * h + r % 4]);
+    for (int j = 0; j < 8; ++j) {
+      const int t = 2 * s + j / 4, r = j % 4;
+      act[s][j] = (__bf16)gelu(acc[t][r] + b[16 * t + 4 * g + r]);
+    }
 }
 
-__global__ void __launch_bounds__(256) k_infer(InferArgs a) {
-  __shared__ float sh[4][32][65];    // outputs, then the changes they stand for
-  __shared__ double shp[4][32][53];  // Y^0.1 of the 52 species
-  __shared__ double shy[4][32][53];  // Y of the 52 species
-  const int wv = threadIdx.x / 64, l = threadIdx.x % 64, h = l / 32, cl = l % 32;
-  const int64_t q = ((int64_t)blockIdx.x * 4 + wv) * 32 + cl;
+__global__ void __launch_bounds__(256) k_network(InferArgs a) {
+  __shared__ __attribute__((aligned(16))) bf16_t slab[2 * 256 * 32];  // weight k-steps, double-buffered
+  const int wv = threadIdx.x / 64, l = threadIdx.x % 64, g = l / 16, cl = l % 16;
+  const int64_t q = ((int64_t)blockIdx.x * 4 + wv) * 16 + cl;
   const bool live = q < a.ncells;
   const int64_t c = live ? a.cells[q] : 0;
   const Norm* nm = a.nm;
-  const Thermo* th = a.th;
-  const double T0 = live ? a.tk[c] * th->tstr : 0.0;
-  // features: this lane supplies k = 16 s + 8 h + j of its cell; Y^0.1 of its species
-  // kept in LDS for the gate
-  bf16x8 x[4];
-  int inside = 1;
+  // features: this lane supplies k = 32 s + 8 g + j of its cell
+  bf16x8 x[2];
 #pragma unroll
-  for (int s = 0; s < 4; ++s)
+  for (int s = 0; s < 2; ++s)
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int f = 16 * s + 8 * h + j;
+      const int f = 32 * s + 8 * g + j;
       float v = 0.f;
       if (live && f < NF) {
         double raw;
         if (f >= 2) {
-          const double yv = a.y[(int64_t)(f - 2) * a.n + c];
-          const double P = bc_base(yv);
-          shy[wv][cl][f - 2] = yv;
-          shp[wv][cl][f - 2] = P;
+          const double P = bc_base(a.y[(int64_t)(f - 2) * a.n + c]);
+          a.pbuf[(int64_t)(f - 2) * a.ldo + q] = P;
           raw = (P - 1.0) / 0.1;
         } else {
-          raw = f == 0 ? T0 : a.log_dt;
+          raw = f == 0 ? a.tk[c] * a.tstr : a.log_dt;
         }
-        inside = inside && raw >= th->xlo[f] && raw <= th->xhi[f];
         v = (float)((raw - nm->xmean[f]) / nm->xsd[f]);
       }
       x[s][j] = (__bf16)v;
     }
-  inside = inside && __shfl_xor(inside, 32);
-  f32x16 acc[8];
-  bf16x8 act[16];
-  infer_layer<8, 4>(a.w[0], x, acc, cl, h);
-  infer_activate(acc, a.b[0], h, act);
+  f32x4 acc[16];
+  bf16x8 act[8];
+  infer_layer<16, 2>(a.w[0], x, acc, cl, g, slab);
+  infer_activate(acc, a.b[0], g, act);
 #pragma unroll 1
   for (int L = 1; L < 4; ++L) {
-    infer_layer<8, 16>(a.w[L], act, acc, cl, h);
-    infer_activate(acc, a.b[L], h, act);
+    infer_layer<16, 8>(a.w[L], act, acc, cl, g, slab);
+    infer_activate(acc, a.b[L], g, act);
   }
-  infer_layer<2, 16>(a.w[4], act, acc, cl, h);
-#pragma unroll
-  for (int t = 0; t < 2; ++t)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int o = 32 * t + 8 * (r / 4) + 4 * h + r % 4;
-      const float v = acc[t][r] + a.b[4][o];
-      sh[wv][cl][o] = v;
-      if (a.out && live) a.out[q * NPAD + o] = v;
-    }
-  __syncthreads();
-  // outputs -> changes, range and slow-chemistry flags: both lanes of a cell, alternate
-  // outputs
-  bool ok = inside != 0, slow = true;
+  infer_layer<4, 8>(a.w[4], act, acc, cl, g, slab);
   if (live)
-    for (int j = h; j < NT; j += 2) {
-      bool r0, r1;
-      sh[wv][cl][j] = out_change(sh[wv][cl][j], j, nm, th, a.tol_bc, r0, r1);
-      ok = ok && r0;
-      slow = slow && r1;
-    }
-  ok = ok && __shfl_xor((int)ok, 32);
-  slow = slow && __shfl_xor((int)slow, 32);
-  __syncthreads();
-  // the gate, one lane per cell
-  int why = -1;
-  double rr[3] = {0.0, 0.0, 0.0};
-  if (live && h == 0) {
-    const float* dv = sh[wv][cl];
-    const double* pv = shp[wv][cl];
-    const double* yv = shy[wv][cl];
-    why = gate_cell([&](int j) { return dv[j]; }, [&](int k) { return pv[k]; }, [&](int k) { return yv[k]; }, th,
-                    a.y, a.tk, a.ht, a.stmf[c], a.stnd[c], a.n, c, ok, slow, a.tol_el, a.tol_sum, a.tol_dt, a.tol_dy,
-                    rr);
-    a.mask[c] = why == 0 ? 10 : a.iskip[c];
-  }
-  gate_stats(why, rr, a.stats);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int o = 16 * t + 4 * g + r;
+        a.out[o * a.ldo + q] = acc[t][r] + a.b[4][o];
+      }
 }
 
 // density of every cell from (Y, T) (main:1056-1061), then the grid's edges: the last
@@ -511,20 +518,24 @@ int us_gate(const float* out, const void* norm, const void* thermo, double* y, d
             double tol_el, double tol_sum, double tol_dt, double tol_dy, double tol_bc, int32_t* reject,
             unsigned long long* stats, void* stream) {
   if (ncells <= 0) return 0;
-  k_gate<<<blocks(ncells, 128), 128, 0, (hipStream_t)stream>>>(out, (const Norm*)norm, (const Thermo*)thermo, y, tk, ht,
-                                                               stmf, stnd, n, cells, ncells, log_dt, tol_el, tol_sum,
-                                                               tol_dt, tol_dy, tol_bc, reject, stats);
+  k_gate<<<blocks(ncells, 256), 256, 0, (hipStream_t)stream>>>(out, NPAD, 1, nullptr, 0, (const Norm*)norm,
+                                                                (const Thermo*)thermo, y, tk, ht, stmf, stnd, n, cells,
+                                                                ncells, log_dt, tol_el, tol_sum, tol_dt, tol_dy, tol_bc,
+                                                                reject, nullptr, nullptr, stats);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-// the fused step (k_infer); w and b: five device pointers each, as packed by
-// um_pack_infer and the network's fp32 biases
+// the surrogate's prediction on a field: k_network (features and MLP; outputs to out,
+// [64][ldo] floats, Y^0.1 to pbuf, [52][ldo] doubles, ldo >= ncells) then k_gate (mask
+// and stats); w and b: five device pointers each, as packed by um_pack_infer and the
+// network's fp32 biases
 int us_infer(const uint16_t* const* w, const float* const* b, const void* norm, const void* thermo, double* y,
-             double* tk, double* ht, const double* stmf, const double* stnd, const int32_t* iskip,
-             int32_t* mask, float* out, int64_t n, const int32_t* cells, int32_t ncells, double log_dt, double tol_el,
-             double tol_sum, double tol_dt, double tol_dy, double tol_bc, unsigned long long* stats, void* stream) {
-  if (ncells < 0 || n < 1) return fail(-1, "us_infer: bad sizes");
+             double* tk, double* ht, const double* stmf, const double* stnd, const int32_t* iskip, int32_t* mask,
+             float* out, double* pbuf, int64_t ldo, int64_t n, const int32_t* cells, int32_t ncells, double tstr,
+             double log_dt, double tol_el, double tol_sum, double tol_dt, double tol_dy, double tol_bc,
+             unsigned long long* stats, void* stream) {
+  if (ncells < 0 || n < 1 || !out || !pbuf || !mask || ldo < ncells) return fail(-1, "us_infer: bad sizes or buffers");
   if (ncells == 0) return 0;
   InferArgs a{};
   for (int i = 0; i < 5; ++i) {
@@ -532,12 +543,13 @@ int us_infer(const uint16_t* const* w, const float* const* b, const void* norm, 
     a.b[i] = b[i];
   }
   a.nm = (const Norm*)norm;
-  a.th = (const Thermo*)thermo;
-  a.y = y; a.tk = tk; a.ht = ht; a.stmf = stmf; a.stnd = stnd;
-  a.iskip = iskip; a.mask = mask; a.out = out; a.cells = cells; a.n = n; a.ncells = ncells;
-  a.log_dt = log_dt; a.tol_el = tol_el; a.tol_sum = tol_sum; a.tol_dt = tol_dt; a.tol_dy = tol_dy; a.tol_bc = tol_bc;
-  a.stats = stats;
-  k_infer<<<blocks(ncells, 128), 256, 0, (hipStream_t)stream>>>(a);
+  a.y = y; a.tk = tk; a.out = out; a.pbuf = pbuf; a.ldo = ldo; a.cells = cells; a.n = n; a.ncells = ncells;
+  a.tstr = tstr; a.log_dt = log_dt;
+  hipStream_t s = (hipStream_t)stream;
+  k_network<<<blocks(ncells, 64), 256, 0, s>>>(a);
+  k_gate<<<blocks(ncells, 256), 256, 0, s>>>(out, 1, ldo, pbuf, ldo, (const Norm*)norm, (const Thermo*)thermo, y, tk,
+                                              ht, stmf, stnd, n, cells, ncells, log_dt, tol_el, tol_sum, tol_dt,
+                                              tol_dy, tol_bc, nullptr, iskip, mask, stats);
   HIPCHK(hipGetLastError());
   return 0;
 }

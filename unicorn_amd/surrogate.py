@@ -90,7 +90,7 @@ def library() -> C.CDLL:
         "us_batch": [v, v, v, i, v, v, v, v, v],
         "us_features": [v, v, i64, v, C.c_int32, C.c_int32, d, d, v, v, v],
         "us_gate": [v, v, v, v, v, v, v, v, i64, v, C.c_int32, d, d, d, d, d, d, v, v, v],
-        "us_infer": [v, v, v, v, v, v, v, v, v, v, v, v, i64, v, C.c_int32, d, d, d, d, d, d, v, v],
+        "us_infer": [v, v, v, v, v, v, v, v, v, v, v, v, v, i64, i64, v, C.c_int32, d, d, d, d, d, d, d, v, v],
         "us_density": [v, v, v, v, i64, i64, v],
         "us_sizes": [v],
     }
@@ -493,12 +493,13 @@ def thermo_block(stats: dict, sc: Scales | None = None) -> Thermo:
 class SurrogateStep:
     """Gated surrogate step with FP64 fallback on one device-resident field.
 
-    fused=True runs features, network and gate in one kernel (us_infer); fused=False
-    runs them as separate kernels (us_features, the training forward pass, us_gate),
-    which the tests use to cross-check the fused kernel."""
+    fused=True runs the features and the whole network in one kernel (activations in
+    registers, weights shared through LDS), then the gate (us_infer); fused=False runs the features, each layer's GEMM and activation as
+    separate kernels (us_features, the training forward pass), then the same gate
+    (us_gate); the tests use it to cross-check the fused network."""
 
     def __init__(self, net: MLP, stats: dict, field, tol_el: float = 1e-6, tol_sum: float = 1e-6,
-                 tol_dt: float = 5.0, tol_dy: float = 2e-3, tol_bc: float = 0.1, fused: bool = True):
+                 tol_dt: float = 5.0, tol_dy: float = 2e-3, tol_bc: float = 0.03, fused: bool = True):
         import torch
 
         from .chem import ChemStep
@@ -517,6 +518,9 @@ class SurrogateStep:
         self.stats = torch.zeros(8, dtype=torch.int64, device="cuda")
         if fused:
             self.wp, self.wptr, self.bptr = net.pack_infer()
+            ld = max(self.ncells, 1)
+            self.out = torch.empty((NPAD, ld), dtype=torch.float32, device="cuda")  # outputs, species-major
+            self.pbuf = torch.empty((LSP, ld), dtype=torch.float64, device="cuda")  # Y^0.1 of the features
         else:
             self.X = torch.zeros((self.bpad, NPAD), dtype=torch.int16, device="cuda")
             self.reject = torch.zeros(field.n, dtype=torch.int32, device="cuda")
@@ -535,10 +539,12 @@ class SurrogateStep:
         if self.fused:
             self.mask.copy_(dev.iskip)
             _ok(self.lib.us_infer(self.wptr, self.bptr, _p(self.nm), _p(self.th), _p(dev.y), _p(dev.tk), _p(dev.ht),
-                                  _p(dev.stmf), _p(dev.stnd), _p(dev.iskip), _p(self.mask),
-                                  None if out is None else _p(out), n, _p(self.cells), self.ncells, log_dt,
-                                  self.tol_el, self.tol_sum, self.tol_dt, self.tol_dy, self.tol_bc, _p(self.stats),
-                                  s))
+                                  _p(dev.stmf), _p(dev.stnd), _p(dev.iskip), _p(self.mask), _p(self.out),
+                                  _p(self.pbuf), self.out.shape[1], n, _p(self.cells), self.ncells, self.sc.tstr,
+                                  log_dt, self.tol_el, self.tol_sum, self.tol_dt, self.tol_dy, self.tol_bc,
+                                  _p(self.stats), s))
+            if out is not None:
+                out.copy_(self.out[:, : self.ncells].t())
         else:
             _ok(self.lib.us_features(_p(dev.y), _p(dev.tk), n, _p(self.cells), self.ncells, self.bpad,
                                      self.sc.tstr, log_dt, _p(self.nm), _p(self.X), s))

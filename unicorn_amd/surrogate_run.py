@@ -118,6 +118,10 @@ def cmd_infer(a):
         "chained_final_abs_dT_K": chained_dT[-1], "chained_final_major_species_error": chained_dY[-1],
         "sumY_residual_fp64": float(sum_ref), "sumY_residual_surrogate": float(sum_sur),
         "ms_per_step_fp64": ms_fp64, "ms_per_step_surrogate": ms_sur,
+        "second_half": {"ms_step": float(np.mean([p["ms_total"] for p in per[a.steps // 2:]])),
+                        "ms_predict": float(np.mean([p["ms_surrogate"] for p in per[a.steps // 2:]])),
+                        "fallback_rate": float(np.mean(rej[a.steps // 2:])),
+                        "steps_with_fallback": int(sum(p["rejected"] > 0 for p in per[a.steps // 2:]))},
         "speedup": ms_fp64 / ms_sur,
         "gate": {"tol_el": a.tol_el, "tol_sum": a.tol_sum, "tol_dT_K": a.tol_dt, "tol_dY": a.tol_dy, "tol_dBC": a.tol_bc},
         "gate_rejections_last_step": {k: per[-1][k] for k in ("out_of_range", "residual", "sum", "active")},
@@ -144,7 +148,7 @@ def main(argv=None):
     i.add_argument("--tol-sum", type=float, default=1e-6)
     i.add_argument("--tol-dt", type=float, default=5.0)
     i.add_argument("--tol-dy", type=float, default=2e-3)
-    i.add_argument("--tol-bc", type=float, default=0.1)
+    i.add_argument("--tol-bc", type=float, default=0.03)
     a = p.parse_args(argv)
     {"train": cmd_train, "infer": cmd_infer}[a.cmd](a)
 
