@@ -13,7 +13,11 @@ the roofline of the dominant kernel k_sweep (FP64, algorithmic operation count
 of the sweeps it executed, timed with HIP events on its stream) and the CPU
 baseline: the reference Fortran (flang build, oracle/_ref) on a sample of the
 same workload -- the same cells' 20 steps with the sweep counts the GPU run
-found -- one process per core.
+found -- one process per core.  With the trained surrogate weights present, the
+line also carries "surrogate": the gated surrogate with FP64 fallback over the same
+K steps from the same seeded state (cell-updates/s, speedup over the FP64 step,
+fallback rate) and its error against the FP64 run (one-step and chained max |dT|
+and max major-species error).
 """
 from __future__ import annotations
 
@@ -47,6 +51,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--cpu-cells", type=int, default=16000, help="CPU baseline sample size (cells)")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-surrogate", action="store_true", help="skip the gated-surrogate measurement")
     p.add_argument("--li", type=int, default=1001)
     p.add_argument("--lj", type=int, default=401)
     p.add_argument("--workload", choices=("field", "batch"), default="field",
@@ -226,6 +231,17 @@ def main():
                    "executed_per_cell_step": sweeps / (ncell * args.steps),
                    "redo_cells": [s.redo_cells for s in stats], "first_segment": [s.first_segment for s in stats]},
     }
+    if not args.no_surrogate:
+        from unicorn_amd.surrogate_run import DEFAULT_WEIGHTS, evaluate
+        if DEFAULT_WEIGHTS.exists():
+            ev = evaluate(f0, args.steps)
+            out["surrogate"] = {k: ev[k] for k in (
+                "cell_updates_per_s", "ms_per_step_surrogate", "ms_per_step_fp64", "speedup", "fallback_rate_mean",
+                "one_step_max_abs_dT_K", "one_step_max_abs_major_species_error", "chained_max_abs_dT_K",
+                "chained_max_abs_major_species_error", "sumY_residual_surrogate", "gate", "second_half")}
+            out["surrogate"]["dtype"] = "bf16 network (fp32 accumulation), fp64 gate and fallback"
+        else:
+            out["surrogate"] = None
     if not args.no_cpu:
         cpu = cpu_baseline(f0, nstars, nhs, args.cpu_cells, args.steps)
         if cpu:
