@@ -1,6 +1,7 @@
-"""The C-ABI library: it loads, exports every function include/unicorn_chem.h
-declares for it, and its host-only entry points agree with the other
-restatements of the reference's scales.  No GPU call is made here."""
+"""The C-ABI libraries: they load and export every function include/unicorn_chem.h
+and include/unicorn_surrogate.h declare for them, and the chemistry library's
+host-only entry points agree with the other restatements of the reference's
+scales.  No GPU call is made here."""
 from __future__ import annotations
 
 import re
@@ -15,12 +16,14 @@ from unicorn_amd.scales import Scales
 
 ROOT = Path(__file__).resolve().parents[1]
 HEADER = ROOT / "include" / "unicorn_chem.h"
+SUR_HEADER = ROOT / "include" / "unicorn_surrogate.h"
+SUR_LIB = ROOT / "unicorn_amd" / "lib" / "libunicorn_surrogate.so"
 
 pytestmark = pytest.mark.skipif(not chem.LIB_PATH.exists(), reason="library not built")
 
 
-def declared_functions():
-    text = HEADER.read_text()
+def declared_functions(header=HEADER):
+    text = header.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\**\s*(\w+)\s*\(", text, flags=re.M)))
 
@@ -36,6 +39,18 @@ def test_library_exports_header():
     assert set(core) == set(chem.EXPORTS), (core, chem.EXPORTS)
     missing = set(core) - exported(chem.LIB_PATH)
     assert not missing, missing
+
+
+def test_surrogate_library_exports_header():
+    if not SUR_LIB.exists():
+        pytest.skip("surrogate library not built")
+    names = declared_functions(SUR_HEADER)
+    assert names and all(n.startswith(("us_", "um_")) for n in names), names
+    missing = set(names) - exported(SUR_LIB)
+    assert not missing, missing
+    # and nothing exported that the header leaves out
+    extra = {n for n in exported(SUR_LIB) if n.startswith(("us_", "um_"))} - set(names)
+    assert not extra, extra
 
 
 def test_fortran_entry_points_declared():
