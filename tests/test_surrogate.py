@@ -400,8 +400,8 @@ def test_gpu_activity_rule(cuda):
 @pytest.mark.gpu
 def test_gpu_surrogate_step_with_fallback(cuda):
     """step() = main:821-828 then predict() on the accepted cells, bitwise, plus, on the
-    rejected ones, bitwise what the FP64 step gives for the field with the accepted
-    cells masked out (ISKIP = 10); the field's own ISKIP is restored."""
+    rejected ones, bitwise what the FP64 step gives for the field with the accepted cells
+    masked out (ISKIP = 10); the field's own ISKIP is restored."""
     torch = cuda
     from unicorn_amd.chem import ChemStep, Controls, DeviceField, clip
     from unicorn_amd.fields import flame_field

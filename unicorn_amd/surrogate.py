@@ -22,7 +22,10 @@ per-cell gate then accepts it only when:
   predicted change of Box-Cox(Y_k) <= ``tol_bc`` (relative: it flags the radical
   build-up before an ignition), so that the network's relative error stays a small
   absolute one (fast cells go to FP64).
-Rejected cells are advanced by the FP64 step (every accepted cell masked out).
+Rejected cells are advanced by the FP64 step (every accepted cell masked out) with the
+caller's controls: the global stop over the rejected cells, normalised by the whole grid
+as the reference normalises it, lands close to the full field's own stop because the
+stiff cells the gate rejects carry most of the field's residual.
 Accepted cells keep their soot scalars, which the network does not predict.  No
 counterpart in the reference.
 """
