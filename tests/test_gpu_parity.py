@@ -102,3 +102,55 @@ def test_gpu_full_size_properties_and_sample(torch_cuda):
             assert not bad.any(), (k, bad.sum())
         else:
             assert np.all(np.abs(a[si] - b[si]) <= 1e-8 * np.abs(b[si]) + 1e-300), k
+
+
+def _holed(field, seed, frac=0.3):
+    """The field with ISKIP in 1..10 at a seeded 30 % of its interior cells (>= 3 skips
+    the cell, ckrates.f:93)."""
+    rng = np.random.default_rng(seed)
+    f = field.copy()
+    c = f.interior()
+    pick = c[rng.random(c.size) < frac]
+    f.iskip[pick] = rng.integers(1, 11, pick.size)
+    return f
+
+
+def test_gpu_iskip_holes_match_oracle(torch_cuda):
+    """Skipped cells scattered over a 61 x 41 C7H16 field: the active-cell list built on
+    the device gives the oracle's two steps (sweep counts exact, the state to the bar on
+    the active cells), and the skipped cells come out as the oracle leaves them."""
+    from unicorn_amd.fields import flame_field
+    f0 = _holed(flame_field(61, 41, "C7H16", 31, preignition_band=0.1), 5)
+    act = f0.active()
+    skip = np.setdiff1d(f0.interior(), act)
+    assert act.size > 0 and skip.size > 0.15 * f0.interior().size
+    ref = f0.copy()
+    outs = [pyoracle.step(ref) for _ in range(2)]
+    dev, stats = gpu_run(f0.copy(), 2)
+    assert [s.isor1 for s in stats] == [o.isor1 for o in outs]
+    assert [s.isor2 for s in stats] == [o.isor2 for o in outs]
+    e = {k: getattr(ref, k) for k in ("rho", "y", "tk", "ht", "stmf", "stnd")}
+    got = dev.to_host()
+    assert_state_close(got, e, act, "iskip holes, active")
+    assert_state_close(got, e, skip, "iskip holes, skipped")
+    for name in ("tk", "ht", "stmf", "stnd"):
+        assert np.array_equal(getattr(got, name)[skip], getattr(f0, name)[skip]), name
+
+
+def test_gpu_device_and_host_cell_lists_agree(torch_cuda):
+    """The same holed field through uc_step (its cell list built on the device) and as
+    the second grid of a two-grid uc_step_batch (the list built on the host): the same
+    results, bit for bit, over two steps."""
+    from unicorn_amd.chem import ChemStep, Controls, DeviceField
+    from unicorn_amd.fields import flame_field
+    f0 = _holed(flame_field(61, 41, "C7H16", 32, preignition_band=0.1), 6)
+    g0 = flame_field(41, 31, "CH4", 7)
+    one, _ = gpu_run(f0.copy(), 2)
+    ctx = ChemStep(f0.n + g0.n, max_grids=2)
+    devs = [DeviceField(g0), DeviceField(f0)]
+    for _ in range(2):
+        ctx.step_batch(devs, [Controls(dt=g0.dt), Controls(dt=f0.dt)])
+    a, b = one.to_host(), devs[1].to_host()
+    for name in ("rho", "y", "tk", "ht", "stmf", "stnd"):
+        assert np.array_equal(getattr(a, name), getattr(b, name)), name
+    assert np.array_equal(one.qdot.cpu().numpy()[f0.active()], devs[1].qdot.cpu().numpy()[f0.active()])
