@@ -359,7 +359,11 @@ struct SourcesArgs {
 
 // Iterates at the decided count, SST2 completion, N2 closure (solvesp.f:2839-2914)
 // and the SOLVEH sources: radiation, RA/PRD, QDOT, QDSP, RHSZ (solveh.f:34-1142).
-__global__ void __launch_bounds__(256) k_sources(const UcParams* __restrict__ Pg, SourcesArgs a) {
+// One wave per block: the block's dynamic LDS (UC_PRD_LDS doubles per lane) is the
+// second-level cache of the RA values uc_prd forms.
+__global__ void __launch_bounds__(WAVE) k_sources(const UcParams* __restrict__ Pg, SourcesArgs a) {
+  extern __shared__ double uc_lds[];
+  double* const lds = uc_lds + threadIdx.x;
   UcP P = (UcP)Pg;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= a.npad) return;
@@ -435,7 +439,7 @@ __global__ void __launch_bounds__(256) k_sources(const UcParams* __restrict__ Pg
   double prod = 0.0, hrr = 0.0;
   const FieldBuf QD = field_buf(a.qdsp, n * 8 * NSP, c);
   const bool hiT = ip != 0;
-  uc_prd(P, rc, s, [&](UcP Q, int k, double pr) {
+  uc_prd(P, rc, s, lds, [&](UcP Q, int k, double pr) {
     // per-species enthalpy HENTH (solveh.f:39-57) with this cell's NASA range
     double pk[6];
 #pragma unroll
@@ -918,6 +922,8 @@ int uc_create_batch(const uc_scales* scales, int64_t max_cells, int32_t max_grid
     HIPCHK(hipFuncSetAttribute((const void*)k_sweep<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                UC_SWEEP_LDS * 512));
   }
+  if (UC_PRD_LDS * 512 > 65536)
+    HIPCHK(hipFuncSetAttribute((const void*)k_sources, hipFuncAttributeMaxDynamicSharedMemorySize, UC_PRD_LDS * 512));
   for (auto& e : ctx->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDefault));
   HIPCHK(hipStreamCreateWithFlags(&ctx->gs, hipStreamNonBlocking));  // default priority
   *out = ctx.release();

@@ -534,16 +534,19 @@ def prod_term(slot, reac, done, k, stale_found, doubled_species):
 
 PRD_CACHE = int(os.environ.get("UC_PRD_CACHE", "64"))
 PRD_PREFETCH = int(os.environ.get("UC_PRD_PREFETCH", "24"))
+PRD_LDS = int(os.environ.get("UC_PRD_LDS", "72"))      # LDS positions (doubles per lane): 4 waves a CU
 
 
 def emit_prd(t, L):
     reac = reactants(t)
     L.append("// net production PRD_k (solveh.f:121-1131) with the final iterates s[0..51]:")
     L.append("// pr[k] = sum over reactions of +-RA (RA = RC*prod S); PRD_k = WM_k HFO_k pr[k].")
-    L.append(f"// Each RA is formed once and kept in a register while among the {PRD_CACHE} soonest")
-    L.append("// reused (the reference also forms RA(r) once, solveh.f:117-669, and sums it).")
+    L.append(f"// Each RA is formed once and kept in one of {PRD_CACHE} registers or the lane's {PRD_LDS} LDS")
+    L.append("// positions until its last use (the reference also forms RA(r) once, solveh.f:117-669,")
+    L.append("// and sums it).  lds: this lane's first LDS position, positions 64 doubles apart.")
+    L.append(f"#define UC_PRD_LDS {PRD_LDS}")
     L.append("template <class RcPtr, class Emit>")
-    L.append("__device__ __forceinline__ void uc_prd(UcP P, RcPtr rc, const double (&s)[54], Emit emit) {")
+    L.append("__device__ __forceinline__ void uc_prd(UcP P, RcPtr rc, const double (&s)[54], double* lds, Emit emit) {")
     prd = t["prd"]
     S = []
     for k in range(1, NSP + 1):
@@ -566,9 +569,9 @@ def emit_prd(t, L):
             S.append(f"{head}acc{k} = {src} " + " ".join(parts[i:i + CHUNK]) + ";")
         S.append(f"emit(P, {k - 1}, acc{k});")
     fac = {sl - 1: "".join(f" * s[{j - 1}]" for j in rs) for sl, rs in reac.items()}
-    n, nl, ng = emit_cached(S, L, PRD_CACHE, PRD_PREFETCH, 0, transform=lambda slot, v: v + fac[slot])
+    n, nl, ng = emit_cached(S, L, PRD_CACHE, PRD_PREFETCH, PRD_LDS, transform=lambda slot, v: v + fac[slot])
     L.append("}")
-    L.append(f"// RA reads: {n} written, {ng} loaded")
+    L.append(f"// RA reads: {n} written, {nl} from LDS, {ng} loaded")
     L.append("")
 
 
