@@ -105,6 +105,8 @@ typedef struct {
 
 #define UCS_MAGIC 0x31534355 /* 'UCS1' */
 #define FLAG_FORCE_SWEEPS 1  /* tolrsp/tolrh ignored: exactly isor1/isor2 sweeps */
+#define FLAG_PROBE_FLUX 2    /* tests of the drop-in's contract: a non-zero VDF at one cell */
+#define FLAG_PROBE_COEF 4    /* tests of the drop-in's contract: a non-zero /DUMMY/ coefficient */
 
 static void rd(void *p, size_t n, FILE *f) { if (fread(p, 1, n, f) != n) die("short read"); }
 static void wr(const void *p, size_t n, FILE *f) { if (fwrite(p, 1, n, f) != n) die("short write"); }
@@ -229,6 +231,8 @@ int main(int argc, char **argv) {
         memset(&dumx_, 0, sizeof dumx_);
         memset(dums_.ste, 0, sizeof dums_.ste); memset(dums_.stw, 0, sizeof dums_.stw);
         memset(dums_.stn, 0, sizeof dums_.stn); memset(dums_.sts, 0, sizeof dums_.sts);
+        if (in.flags & FLAG_PROBE_FLUX) cb07_.vdf[0][fidx(1, 1)] = 1.0e-5;
+        if (in.flags & FLAG_PROBE_COEF) dummy_.a[0][fidx(1, 1)] = 1.0e-3;
         int32_t isor1 = (int32_t)in.isor1, isor2 = (int32_t)in.isor2;
         double relxsp = in.relxsp, tolrsp = in.tolrsp, relxh = in.relxh, tolrh = in.tolrh;
         double spn = spnorm, hn = hnorm;

@@ -21,6 +21,8 @@ DRIVER = HERE / "_ref" / "ref_driver"
 SHIM_DRIVER = HERE / "_ref" / "ref_driver_shim"
 LSP, LRX = 52, 544
 FLAG_FORCE_SWEEPS = 1
+FLAG_PROBE_FLUX = 2   # contract tests of the drop-in: a non-zero VDF at one cell
+FLAG_PROBE_COEF = 4   # contract tests of the drop-in: a non-zero /DUMMY/ coefficient
 
 
 def available(driver: str = "ref") -> bool:
@@ -50,14 +52,14 @@ class RefResult:
 
 
 def write_input(path, field, nsteps=1, isor1=100, isor2=100, relxsp=0.99, tolrsp=1e-8,
-                relxh=0.99, tolrh=1e-8, force=False, dump_cells=(), ref_values=None):
+                relxh=0.99, tolrh=1e-8, force=False, dump_cells=(), ref_values=None, probe=0):
     from unicorn_amd.scales import Scales
     sc = Scales()
     rv = ref_values or [sc.vref, sc.tref, sc.pref, sc.rtin, sc.rtot, sc.fo2in]
     dump_cells = np.asarray(dump_cells, dtype=np.int64)
     with open(path, "wb") as f:
         hdr = np.array([0x31534355, field.lj, field.li, nsteps, isor1, isor2,
-                        FLAG_FORCE_SWEEPS if force else 0, len(dump_cells)], dtype="<i8")
+                        (FLAG_FORCE_SWEEPS if force else 0) | probe, len(dump_cells)], dtype="<i8")
         f.write(hdr.tobytes())
         f.write(np.asarray(rv, dtype="<f8").tobytes())
         f.write(np.array([field.dt, relxsp, tolrsp, relxh, tolrh], dtype="<f8").tobytes())

@@ -55,6 +55,21 @@ def test_shim_without_gpu_stops_with_a_message():
     assert "unicorn_fshim: ckrates_:" in e.value.stderr
 
 
+@pytest.mark.parametrize("probe,message", [(refrun.FLAG_PROBE_FLUX, "species-diffusion enthalpy flux"),
+                                           (refrun.FLAG_PROBE_COEF, "non-zero transport coefficient in /DUMMY/")])
+def test_shim_refuses_transport_before_the_gpu(probe, message):
+    """The drop-in implements the chemistry-only contract: a non-zero VDF (with SPGSOLV's
+    non-zero EMU) or a non-zero /DUMMY/ coefficient at one interior cell stops the
+    program with a message, before any GPU call, GPU or not."""
+    if not refrun.available("shim"):
+        pytest.skip("ref_driver_shim not built")
+    f0, _ = load("sooty")
+    with pytest.raises(subprocess.CalledProcessError) as e:
+        refrun.run(f0, nsteps=1, driver="shim", probe=probe)
+    assert e.value.returncode == 3
+    assert message in e.value.stderr, e.value.stderr
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", CASES)
 def test_shim_matches_reference_fixtures(name):

@@ -58,7 +58,8 @@ extern struct { double rsoot, stmf[shim::LE], stnd[shim::LE]; } soot_;
 extern struct { double a[55][shim::LE]; } dummy_;
 extern struct { double a[149][shim::LE]; } dum0_;
 extern struct { double z[4][shim::LE]; } dumx_;
-extern struct { double ssp[shim::LSP][shim::LE], fpz[shim::LE]; } dumy_;
+extern struct { double ssp[shim::LSP][shim::LE], fpz[shim::LE], emu[shim::LE], ekt[shim::LE]; } dumy_;
+extern struct { double vmu[shim::LE], vtc[shim::LE], vdf[shim::LSP][shim::LE], tmu[shim::LE]; } cb07_;
 extern struct { double st[4][shim::LE], sst1[shim::LE], sst2[shim::LE]; } dums_;
 }
 
@@ -154,6 +155,18 @@ void check_contract() {
     if (nonzero(dumx_.z[k])) stop("ckrates_", "non-zero enthalpy coefficient in /DUMX/");
     if (nonzero(dums_.st[k])) stop("ckrates_", "non-zero soot coefficient in /DUMS/");
   }
+  // RDHSP = (EMU VDF + REI TMU / SIGSP - EKT) HSSP (solveh.f:52-53) feeds the species-
+  // diffusion enthalpy flux of RHSZ (solveh.f:59-115) through every cell's neighbours;
+  // this drop-in computes RHSZ with that flux zero, so RDHSP must vanish everywhere:
+  // EKT = TMU = 0, and VDF = 0 wherever EMU is not (SPGSOLV sets EMU even without
+  // transport)
+  for (int64_t c = 0; c < shim::LE; ++c) {
+    bool flux = dumy_.ekt[c] != 0.0 || cb07_.tmu[c] != 0.0;
+    for (int k = 0; k < shim::LSP && !flux && dumy_.emu[c] != 0.0; ++k) flux = cb07_.vdf[k][c] != 0.0;
+    if (flux)
+      stop("ckrates_", "non-zero species-diffusion enthalpy flux input (EKT, TMU, or VDF with EMU): the flux "
+                       "term of SOLVEH is not implemented, only the chemistry-only contract");
+  }
 }
 
 void h2d(void* dst, const void* src, size_t b, const char* where) {
@@ -169,8 +182,8 @@ extern "C" {
 
 // SUBROUTINE CKRATES (src/lib/ckrates.f:1)
 void ckrates_(void) {
+  check_contract();  // host COMMONs only: refuse before touching the GPU
   ensure_context();
-  check_contract();
   const size_t b1 = sizeof(double) * shim::LE, b52 = b1 * shim::LSP;
   h2d(g.rho, cb04_.rho, b1, "ckrates_");
   h2d(g.y, cb04_.fsp, b52, "ckrates_");  // FA = SA on entry (spgsolv.f:169-220)
