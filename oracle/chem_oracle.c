@@ -3,8 +3,12 @@
  * and only as the checker.
  *
  * CPU restatement of UNICORN's chemistry step (the CKRATES -> SOLVESP -> SOLVEH
- * chain that SPGSOLV calls, src/lib/spgsolv.f:1293-1298), for the
- * chemistry-only contract (all transport coefficients zero).  Table driven
+ * chain that SPGSOLV calls, src/lib/spgsolv.f:1293-1298).  uco_step: the
+ * chemistry-only contract (all transport coefficients zero), cells in parallel.
+ * uco_step_cpl: with the coefficients SPGSOLV assembles (E/W/P/Q of every species,
+ * STE..STS, ZE..ZS), the species-diffusion enthalpy flux of RHSZ and the kinetic
+ * energy in T(h): the reference's lexicographic Gauss-Seidel (J outer, I inner),
+ * serial.  Table driven
  * from oracle/mech_tables.h (generated from unicorn_amd/mech/sd52.json).  Each
  * arithmetic expression is evaluated in the reference's left-to-right order,
  * with its float32 literals, so results can be compared bit for bit with the
@@ -44,7 +48,26 @@ typedef struct {
     int64_t newton_iters;        /* sum over cells of Newton iterations */
 } uco_stats;
 
-int uco_abi_version(void) { return 1; }
+/* Inputs beyond the chemistry state (any pointer may be NULL: zero).  n = li*lj, field
+ * layout as above.  coef[4m+f][n]: face f (E, W, P, Q) of species m+1 in the order of
+ * /DUMMY/ then /DUM0/ (solvesp.f:24-58: EA0, WA0, PA0, QA0, EA1, ...).  P couples J+1,
+ * Q J-1, E I+1, W I-1 (solvesp.f:425-426). */
+typedef struct {
+    const double *coef;            /* [204][n] */
+    const double *stc;             /* [4][n] STE, STW, STN, STS (soot, solvesp.f:2812-2825) */
+    const double *zc;              /* [4][n] ZE, ZW, ZN, ZS (enthalpy, solveh.f:1155-1158) */
+    const double *emu;             /* [n]; NULL: REI*BETA3*RHO as SPGSOLV sets it (spgsolv.f:175) */
+    const double *ekt, *tmu;       /* [n] */
+    const double *vdf;             /* [52][n] */
+    const double *ygeo;            /* [n] Y(J,I) (/CB03/) */
+    const double *xxc, *xxs;       /* [li] */
+    const double *yyc, *yys;       /* [lj] */
+    const double *u, *v, *w;       /* [n] velocities (T(h), solveh.f:1176-1177) */
+    double sigsp;                  /* SOLVEH's SIGSP argument */
+    int32_t isym;                  /* /CB01/ ISYM */
+} uco_coupling;
+
+int uco_abi_version(void) { return 2; }
 
 /* ------------------------------------------------------------------ CKRATES */
 /* One cell of src/lib/ckrates.f:95-1352.  fa: old mass fractions (52). */
@@ -156,12 +179,14 @@ static void ck_cell(const uc_scales *S, double tk, double rho, const double *fa,
 }
 
 /* ------------------------------------------------------------------ SOLVESP */
-/* One cell-sweep of src/lib/solvesp.f:91-2831 (chemistry-only: E/W/P/Q = 0).
- * s[0..51] species iterates (SA*), s[52]=SST1, s[53]=SST2.  d[0..51] receives
- * |S-FNEW| per species 1..51 then SST1, in the order RSD accumulates them. */
+/* One cell-sweep of src/lib/solvesp.f:91-2831.  s[0..51] species iterates (SA*),
+ * s[52]=SST1, s[53]=SST2.  d[0..51] receives |S-FNEW| per species 1..51 then SST1,
+ * in the order RSD accumulates them.  nb (NULL: chemistry-only, E/W/P/Q = 0): per
+ * relaxed species m (0..50) nine values {PPP, P, S(J+1), Q, S(J-1), E, S(I+1), W,
+ * S(I-1)}, PPP = RHO-(E+W+P+Q) (solvesp.f:424-426). */
 static void sp_cell_sweep(const double *rc, const double *sour, const double *fa, double rho,
                           double stmf, double stnd, double dt, double sor1, double sor2,
-                          const double *wm, double *s, double *d) {
+                          const double *wm, double *s, double *d, const double *nb) {
     double A[NRX];
     double rhs1a = 0.0, rhs1b = 0.0, loss = 0.0;
     int k = -1, nd = 0;
@@ -198,8 +223,16 @@ static void sp_cell_sweep(const double *rc, const double *sour, const double *fa
         case 9: {
             double rhs1 = rho * fa[k] + dt * wm[k] * (rhs1a + rhs1b);
             double rhs2 = (k >= 49) ? 0.0 : -dt * wm[k] * loss;
-            double ppp = rho - 0.0;
-            double fnew = sor2 * s[k] + rhs1 * sor1 / (ppp - rhs2);
+            double ppp = rho - 0.0, num = rhs1;
+            if (nb) { /* RHS1-P*S(J+1)-Q*S(J-1)-E*S(I+1)-W*S(I-1), left to right */
+                const double *b = &nb[9 * k];
+                ppp = b[0];
+                num = num - b[1] * b[2];
+                num = num - b[3] * b[4];
+                num = num - b[5] * b[6];
+                num = num - b[7] * b[8];
+            }
+            double fnew = sor2 * s[k] + num * sor1 / (ppp - rhs2);
             d[nd++] = fabs(s[k] - fnew);
             if (fnew < 0.0) fnew = 0.0;
             s[k] = fnew;
@@ -210,26 +243,40 @@ static void sp_cell_sweep(const double *rc, const double *sour, const double *fa
     (void)stmf; (void)stnd;
 }
 
-/* Soot mass fraction and number density, solvesp.f:2807-2831 (SST1 feeds RSD, SST2 does not). */
+/* RHS1 minus the neighbour terms of nb's nine values (see sp_cell_sweep), left to right */
+static double sp_num(double rhs1, const double *b) {
+    double num = rhs1;
+    num = num - b[1] * b[2];
+    num = num - b[3] * b[4];
+    num = num - b[5] * b[6];
+    num = num - b[7] * b[8];
+    return num;
+}
+
+/* Soot mass fraction and number density, solvesp.f:2807-2831 (SST1 feeds RSD, SST2 does not).
+ * nb as in sp_cell_sweep (NULL: zero coefficients): nb[9*51..] for SST1 and nb[9*52..] for
+ * SST2, both with STE..STS. */
 static void sp_cell_soot(const double *sour, double rho, double stmf, double stnd, double dt,
-                         double sor1, double sor2, double wmstr, double *s, double *d_sst1) {
+                         double sor1, double sor2, double wmstr, double *s, double *d_sst1,
+                         const double *nb) {
     double rhs1 = rho * stmf
                   + dt * (F32(0.012011) / wmstr) * (2.0 * sour[0] * s[21] - sour[2] * s[1]
                                                     - sour[3] * s[4] - sour[4] * s[3]);
-    double ppp = rho - 0.0;
-    double fnew = sor2 * s[52] + rhs1 * sor1 / (ppp - 0.0);
+    double ppp = nb ? nb[9 * 51] : rho - 0.0;
+    double fnew = sor2 * s[52] + (nb ? sp_num(rhs1, &nb[9 * 51]) : rhs1) * sor1 / (ppp - 0.0);
     *d_sst1 = fabs(s[52] - fnew);
     if (fnew < 0.0) fnew = 0.0;
     s[52] = fnew;
     rhs1 = rho * stnd + dt * sour[1] * s[21];
     double rhs2 = -dt * sour[5];
-    fnew = sor2 * s[53] + rhs1 * sor1 / (ppp - rhs2);
+    fnew = sor2 * s[53] + (nb ? sp_num(rhs1, &nb[9 * 52]) : rhs1) * sor1 / (ppp - rhs2);
     if (fnew < 0.0) fnew = 0.0;
     s[53] = fnew;
 }
 
 /* ------------------------------------------------------------------ SOLVEH */
-static void sh_hssp(const uc_scales *S, double tk, double *henth) { /* solveh.f:34-57 */
+/* HENTH and HSSP of one cell (solveh.f:34-57); hssp may be NULL */
+static void sh_hssp(const uc_scales *S, double tk, double *henth, double *hssp_out) {
     const double hconvr = S->gasc / S->wmstr / S->hstr;
     const double tkd = tk * S->tstr, tkd2 = tkd * tkd, tkd3 = tkd * tkd2, tkd4 = tkd * tkd3,
                  tkd5 = tkd * tkd4;
@@ -239,12 +286,14 @@ static void sh_hssp(const uc_scales *S, double tk, double *henth) { /* solveh.f:
         double hssp = (p[0] * tkd + p[1] * tkd2 / 2.0 + p[2] * tkd3 / 3.0 + p[3] * tkd4 / 4.0
                        + p[4] * tkd5 / 5.0) / S->wm[k] * hconvr;
         henth[k] = hssp * S->wmstr * S->hstr + p[5] / S->wm[k] * S->gasc;
+        if (hssp_out) hssp_out[k] = hssp;
     }
 }
 
-/* RHSZ source with zero transport (solveh.f:59-115): RHSZ = -QRAD*DT. */
+/* RHSZ source (solveh.f:59-115): RHSZ = FLUX - QRAD*DT, FLUX = DT*0.5*(species-diffusion
+ * enthalpy flux divergence), zero with zero transport. */
 static double sh_radiation(const uc_scales *S, double tk, double rho, double stmf, const double *ssp,
-                           double dt) {
+                           double dt, double flux) {
     double tmole = 0.0;
     for (int k = 0; k < NSP; ++k) tmole = tmole + ssp[k] / S->wm[k];
     const double tkd = tk * S->tstr;
@@ -266,7 +315,7 @@ static double sh_radiation(const uc_scales *S, double tk, double rho, double stm
     double t5 = tkd * tkd * tkd * tkd * tkd;
     double qrad = crad * F32(0.4) * (4.0 * 5.669e-08 * plankc * (pow(tkd, 4.0) - 8.1e+09)
                                      + 3.334e-04 * stmf * (rho / RSOOT) * t5);
-    return 0.0 - qrad * dt;
+    return flux - qrad * dt;
 }
 
 /* RA, PRD, heat release (solveh.f:117-1145). Returns RHSZ increment pieces. */
@@ -299,9 +348,9 @@ static void sh_rates(const uc_scales *S, const double *rc, const double *ssp, co
     *qdot = -hrr * rconv;
 }
 
-/* T from h by Newton (solveh.f:1170-1230).  Returns Newton iterations. */
-static int sh_temperature(const uc_scales *S, double fpz, const double *ssp, double *tk) {
-    double enth = (fpz - 0.5 * 0.0) * S->hstr;
+/* T from h by Newton (solveh.f:1170-1230).  uu = U*U+V*V+W*W.  Returns Newton iterations. */
+static int sh_temperature(const uc_scales *S, double fpz, double uu, const double *ssp, double *tk) {
+    double enth = (fpz - 0.5 * uu) * S->hstr;
     for (int k = 0; k < NSP; ++k) enth = enth + ssp[k] * S->hfo[k] * S->hstr;
     int ipl = 0;
     double tpoly = MT_TPOL1;
@@ -373,22 +422,63 @@ double uco_temperature(const double *ref, double fpz, const double *y, double tk
     double sq[54];
     for (int k = 0; k < NSP; ++k) sq[k] = y[k];
     sq[52] = sq[53] = 0.0;
-    sh_temperature(&S, fpz, sq, &tk);
+    sh_temperature(&S, fpz, 0.0, sq, &tk);
     return tk;
 }
 
-/* One chemistry step on a field (see oracle/ref_driver.c for the sequence).
- * rc_dump (optional, ndump x (544+6)) receives RC and SOUR of dump_cells after CKRATES.
- * rsd_hist (optional, isor1 doubles) receives RSD/SPNORM after each species sweep.
- * cell_sweeps (optional, n ints) receives the species sweeps each cell executed. */
-int uco_step(const uco_params *p, int li, int lj, double *rho, double *y, double *tk, double *ht,
-             double *stmf, double *stnd, const int32_t *iskip, double *qdot, double *qdsp,
-             uco_stats *st, double *rc_dump, const int64_t *dump_cells, int ndump,
-             double *rsd_hist, int32_t *cell_sweeps) {
+/* Neighbour values of one cell for sp_cell_sweep/sp_cell_soot (nb[9*53]) from the field-layout
+ * iterates sg[54][n] and the coefficients (NULL: zero). */
+static void sp_gather(const uco_coupling *cp, const double *sg, int64_t n, int lj, int64_t c, double rho,
+                      double *nb) {
+    const int64_t cn = c + 1, cs = c - 1, ce = c + lj, cw = c - lj; /* J+1, J-1, I+1, I-1 */
+    for (int m = 0; m < 53; ++m) { /* species 1..51, then SST1 and SST2 (iterate rows 52, 53) */
+        const int64_t row = (int64_t)(m < 51 ? m : m + 1) * n;
+        double e = 0.0, w = 0.0, pp = 0.0, q = 0.0;
+        const double *cf = m < 51 ? (cp->coef ? cp->coef + (int64_t)4 * m * n : NULL) : cp->stc;
+        if (cf) { e = cf[c]; w = cf[n + c]; pp = cf[2 * n + c]; q = cf[3 * n + c]; }
+        double *b = &nb[9 * m];
+        b[0] = rho - (e + w + pp + q);
+        b[1] = pp; b[2] = sg[row + cn];
+        b[3] = q;  b[4] = sg[row + cs];
+        b[5] = e;  b[6] = sg[row + ce];
+        b[7] = w;  b[8] = sg[row + cw];
+    }
+}
+
+/* DT*0.5 times the species-diffusion enthalpy flux divergence of one interior cell
+ * (solveh.f:63-115): rd[52][n] = RDHSP, ssp[52][n] = the species after the closure. */
+static double sh_flux(const uco_coupling *cp, const double *rd, const double *ssp, int64_t n, int lj, int i,
+                      int j, double dt) {
+    const int64_t ia = (int64_t)i * lj + j;
+    double rdhe = 0.0, rdhw = 0.0, rdhn = 0.0, rdhs = 0.0, rdhp = 0.0;
+    for (int k = 0; k < NSP; ++k) {
+        const double *r = &rd[(int64_t)k * n], *y = &ssp[(int64_t)k * n];
+        rdhe = rdhe + (r[ia + lj] + r[ia]) * (y[ia + lj] - y[ia]);
+        rdhw = rdhw + (r[ia] + r[ia - lj]) * (y[ia] - y[ia - lj]);
+        rdhn = rdhn + (r[ia + 1] + r[ia]) * (y[ia + 1] - y[ia]);
+        rdhs = rdhs + (r[ia] + r[ia - 1]) * (y[ia] - y[ia - 1]);
+        rdhp = rdhp + (r[ia]) * (y[ia + 1] - y[ia - 1]);
+    }
+    const double sym = (double)cp->isym;
+    const double x = (rdhe / cp->xxs[i + 1] - rdhw / cp->xxs[i]) / cp->xxc[i]
+                     + (rdhn / cp->yys[j + 1] - rdhs / cp->yys[j]) / cp->yyc[j]
+                     + sym * rdhp / cp->ygeo[ia] / cp->yyc[j];
+    return dt * 0.5 * x;
+}
+
+static int step_impl(const uco_params *p, int li, int lj, double *rho, double *y, double *tk, double *ht,
+                     double *stmf, double *stnd, const int32_t *iskip, double *qdot, double *qdsp,
+                     const uco_coupling *cp, uco_stats *st, double *rc_dump, const int64_t *dump_cells,
+                     int ndump, double *rsd_hist, int32_t *cell_sweeps) {
     uc_scales S;
     uc_scales_init(&S, p->ref[0], p->ref[1], p->ref[2], p->ref[3], p->ref[4], p->ref[5], 1);
     uc_scales_hfo(&S, MT_NASA);
     const int64_t n = (int64_t)li * lj;
+    const int cpl_sp = cp && (cp->coef || cp->stc);
+    const int cpl_h = cp && cp->zc;
+    const int flux = cp && (cp->vdf || cp->tmu || cp->ekt);
+    if ((cpl_sp || cpl_h) && p->mode == 2) return -1; /* per-cell stops need independent cells */
+    if (flux && !(cp->ygeo && cp->xxc && cp->xxs && cp->yyc && cp->yys)) return -2;
     const double spnorm = (double)n * (double)(NSP - 1), hnorm = (double)n;
     /* main:821-828 */
     for (int64_t c = 0; c < n; ++c) {
@@ -415,6 +505,15 @@ int uco_step(const uco_params *p, int li, int lj, double *rho, double *y, double
     unsigned char *frozen = calloc((size_t)nint, 1);
     int32_t *nsw = calloc((size_t)nint, sizeof(int32_t));
     const double sor1 = p->relxsp, sor2 = 1.0 - sor1;
+    /* SA=FA, SST1=STMF, SST2=STND at every cell (spgsolv.f:176-231): the neighbours'
+     * values of the coupled sweeps */
+    double *sg = NULL;
+    if (cpl_sp || flux) {
+        sg = malloc(sizeof(double) * (size_t)n * 54);
+        memcpy(sg, y, sizeof(double) * (size_t)n * NSP);
+        memcpy(sg + 52 * n, stmf, sizeof(double) * (size_t)n);
+        memcpy(sg + 53 * n, stnd, sizeof(double) * (size_t)n);
+    }
 #pragma omp parallel for schedule(dynamic, 16) num_threads(p->nthreads > 0 ? p->nthreads : 1)
     for (int64_t q = 0; q < nint; ++q) {
         const int64_t c = cells[q];
@@ -434,32 +533,46 @@ int uco_step(const uco_params *p, int li, int lj, double *rho, double *y, double
     /* SOLVESP sweeps with the global (or per-cell) stop */
     int isorp, converged = 0;
     for (isorp = 1; isorp <= p->isor1; ++isorp) {
-#pragma omp parallel for schedule(dynamic, 16) num_threads(p->nthreads > 0 ? p->nthreads : 1)
-        for (int64_t q = 0; q < nint; ++q) {
-            double *d = &dl[q * NSP];
-            if (frozen[q]) {
-                memset(d, 0, sizeof(double) * NSP);
-                if (frozen[q] == 1) { /* S fixed: only SST2 keeps relaxing (solvesp.f:2820-2831) */
-                    const int64_t c = cells[q];
-                    double *sq = &s[q * 54];
-                    double dd;
-                    double keep1 = sq[52];
-                    sp_cell_soot(&sour[q * 6], rho[c], stmf[c], stnd[c], p->dt, sor1, sor2, S.wmstr, sq, &dd);
-                    sq[52] = keep1;
-                }
-                continue;
+        if (cpl_sp) { /* the reference's order: each cell reads its neighbours' latest values */
+            for (int64_t q = 0; q < nint; ++q) {
+                const int64_t c = cells[q];
+                double *sq = &s[q * 54], *d = &dl[q * NSP], nb[9 * 53];
+                sp_gather(cp, sg, n, lj, c, rho[c], nb);
+                sp_cell_sweep(&rc[q * NRX], &sour[q * 6], &fa[q * NSP], rho[c], stmf[c], stnd[c], p->dt,
+                              sor1, sor2, S.wm, sq, d, nb);
+                sp_cell_soot(&sour[q * 6], rho[c], stmf[c], stnd[c], p->dt, sor1, sor2, S.wmstr, sq, &d[51], nb);
+                for (int k = 0; k < 54; ++k) sg[(int64_t)k * n + c] = sq[k];
+                nsw[q]++;
             }
-            const int64_t c = cells[q];
-            double *sq = &s[q * 54];
-            sp_cell_sweep(&rc[q * NRX], &sour[q * 6], &fa[q * NSP], rho[c], stmf[c], stnd[c], p->dt,
-                          sor1, sor2, S.wm, sq, d);
-            sp_cell_soot(&sour[q * 6], rho[c], stmf[c], stnd[c], p->dt, sor1, sor2, S.wmstr, sq, &d[51]);
-            nsw[q]++;
-            double sum = 0.0;
-            int zero = 1;
-            for (int k = 0; k < NSP; ++k) { sum += d[k]; zero &= (d[k] == 0.0); }
-            if (p->mode == 1 && zero) frozen[q] = 1;
-            if (p->mode == 2 && sum / (double)(NSP - 1) <= p->tolrsp) frozen[q] = 2;
+        } else {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(p->nthreads > 0 ? p->nthreads : 1)
+            for (int64_t q = 0; q < nint; ++q) {
+                double *d = &dl[q * NSP];
+                if (frozen[q]) {
+                    memset(d, 0, sizeof(double) * NSP);
+                    if (frozen[q] == 1) { /* S fixed: only SST2 keeps relaxing (solvesp.f:2820-2831) */
+                        const int64_t c = cells[q];
+                        double *sq = &s[q * 54];
+                        double dd;
+                        double keep1 = sq[52];
+                        sp_cell_soot(&sour[q * 6], rho[c], stmf[c], stnd[c], p->dt, sor1, sor2, S.wmstr, sq, &dd,
+                                     NULL);
+                        sq[52] = keep1;
+                    }
+                    continue;
+                }
+                const int64_t c = cells[q];
+                double *sq = &s[q * 54];
+                sp_cell_sweep(&rc[q * NRX], &sour[q * 6], &fa[q * NSP], rho[c], stmf[c], stnd[c], p->dt,
+                              sor1, sor2, S.wm, sq, d, NULL);
+                sp_cell_soot(&sour[q * 6], rho[c], stmf[c], stnd[c], p->dt, sor1, sor2, S.wmstr, sq, &d[51], NULL);
+                nsw[q]++;
+                double sum = 0.0;
+                int zero = 1;
+                for (int k = 0; k < NSP; ++k) { sum += d[k]; zero &= (d[k] == 0.0); }
+                if (p->mode == 1 && zero) frozen[q] = 1;
+                if (p->mode == 2 && sum / (double)(NSP - 1) <= p->tolrsp) frozen[q] = 2;
+            }
         }
         double rsd = 0.0; /* RSD in the reference's accumulation order */
         for (int64_t q = 0; q < nint; ++q)
@@ -486,8 +599,24 @@ int uco_step(const uco_params *p, int li, int lj, double *rho, double *y, double
             sq[51] = 0.0;
         }
         y[(NSP - 1) * n + cells[q]] = sq[51]; /* FCB = SCB */
+        if (sg)
+            for (int k = 0; k < 54; ++k) sg[(int64_t)k * n + cells[q]] = sq[k];
     }
-    /* SOLVEH */
+    /* SOLVEH: RDHSP at every cell (solveh.f:34-57), then the sources */
+    double *rd = NULL;
+    if (flux) {
+        rd = malloc(sizeof(double) * (size_t)n * NSP);
+        for (int64_t c = 0; c < n; ++c) {
+            double henth[NSP], hssp[NSP];
+            sh_hssp(&S, tk[c], henth, hssp);
+            const double emu = cp->emu ? cp->emu[c] : S.rei * S.beta3 * rho[c];
+            const double tmu = cp->tmu ? cp->tmu[c] : 0.0, ekt = cp->ekt ? cp->ekt[c] : 0.0;
+            for (int k = 0; k < NSP; ++k) {
+                const double vdf = cp->vdf ? cp->vdf[(int64_t)k * n + c] : 0.0;
+                rd[(int64_t)k * n + c] = (emu * vdf + S.rei * tmu / cp->sigsp - ekt) * hssp[k];
+            }
+        }
+    }
     double *rhsz = malloc(sizeof(double) * (size_t)nint), *fpz = malloc(sizeof(double) * (size_t)nint);
     int64_t newton = 0;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(p->nthreads > 0 ? p->nthreads : 1)
@@ -495,8 +624,9 @@ int uco_step(const uco_params *p, int li, int lj, double *rho, double *y, double
         const int64_t c = cells[q];
         double henth[NSP], qd[NSP];
         const double *sq = &s[q * 54];
-        sh_hssp(&S, tk[c], henth);
-        double r = sh_radiation(&S, tk[c], rho[c], stmf[c], sq, p->dt);
+        sh_hssp(&S, tk[c], henth, NULL);
+        const double fx = flux ? sh_flux(cp, rd, sg, n, lj, (int)(c / lj), (int)(c % lj), p->dt) : 0.0;
+        double r = sh_radiation(&S, tk[c], rho[c], stmf[c], sq, p->dt, fx);
         sh_rates(&S, &rc[q * NRX], sq, henth, rho[c], ht[c], p->dt, &r, &qdot[c], qd);
         for (int k = 0; k < NSP; ++k) qdsp[k * n + c] = qd[k];
         rhsz[q] = r;
@@ -505,17 +635,35 @@ int uco_step(const uco_params *p, int li, int lj, double *rho, double *y, double
     const double zsor1 = p->relxh, zsor2 = 1.0 - zsor1;
     int isorz, zconv = 0;
     unsigned char *zfrozen = calloc((size_t)nint, 1);
+    double *fg = NULL; /* FPZ = HT at every cell (spgsolv.f:176): the coupled sweeps' neighbours */
+    if (cpl_h) {
+        fg = malloc(sizeof(double) * (size_t)n);
+        memcpy(fg, ht, sizeof(double) * (size_t)n);
+    }
     for (isorz = 1; isorz <= p->isor2; ++isorz) {
         double rsdz = 0.0;
         for (int64_t q = 0; q < nint; ++q) {
             if (zfrozen[q]) continue;
             const int64_t c = cells[q];
-            double ppp = rho[c] - 0.0;
-            double fnew = zsor2 * fpz[q] + rhsz[q] * zsor1 / ppp;
+            double fnew;
+            if (cpl_h) { /* solveh.f:1154-1158 */
+                const double ze = cp->zc[c], zw = cp->zc[n + c], zn = cp->zc[2 * n + c], zs = cp->zc[3 * n + c];
+                double ppp = rho[c] - (ze + zw + zn + zs);
+                double num = rhsz[q];
+                num = num - zn * fg[c + 1];
+                num = num - zs * fg[c - 1];
+                num = num - ze * fg[c + lj];
+                num = num - zw * fg[c - lj];
+                fnew = zsor2 * fpz[q] + num * zsor1 / ppp;
+            } else {
+                double ppp = rho[c] - 0.0;
+                fnew = zsor2 * fpz[q] + rhsz[q] * zsor1 / ppp;
+            }
             double dz = fabs(fpz[q] - fnew);
             rsdz = rsdz + dz;
             fpz[q] = fnew;
-            if (p->mode == 1 && dz == 0.0) zfrozen[q] = 1;
+            if (fg) fg[c] = fnew;
+            if (!cpl_h && p->mode == 1 && dz == 0.0) zfrozen[q] = 1;
             if (p->mode == 2 && dz <= p->tolrh) zfrozen[q] = 1;
         }
         if (p->mode != 2 && rsdz / hnorm <= p->tolrh) { zconv = 1; break; }
@@ -529,7 +677,12 @@ int uco_step(const uco_params *p, int li, int lj, double *rho, double *y, double
 #pragma omp parallel for schedule(dynamic, 16) num_threads(p->nthreads > 0 ? p->nthreads : 1) reduction(+ : newton)
     for (int64_t q = 0; q < nint; ++q) {
         const int64_t c = cells[q];
-        newton += sh_temperature(&S, fpz[q], &s[q * 54], &tk[c]);
+        double uu = 0.0;
+        if (cp && (cp->u || cp->v || cp->w)) {
+            const double u = cp->u ? cp->u[c] : 0.0, v = cp->v ? cp->v[c] : 0.0, w = cp->w ? cp->w[c] : 0.0;
+            uu = u * u + v * v + w * w;
+        }
+        newton += sh_temperature(&S, fpz[q], uu, &s[q * 54], &tk[c]);
     }
     /* SPGSOLV epilogue :1299-1416 (I outer, J inner) */
     double resdh = 0.0, resdc = 0.0;
@@ -570,5 +723,27 @@ int uco_step(const uco_params *p, int li, int lj, double *rho, double *y, double
     memcpy(rho, rhonp, sizeof(double) * (size_t)n);
     free(rhonp); free(pos); free(zfrozen); free(rhsz); free(fpz); free(nsw); free(frozen);
     free(dl); free(s); free(fa); free(sour); free(rc); free(cells);
+    free(sg); free(rd); free(fg);
     return 0;
+}
+
+/* One chemistry step on a field (see oracle/ref_driver.c for the sequence), chemistry-only
+ * contract.  rc_dump (optional, ndump x (544+6)) receives RC and SOUR of dump_cells after
+ * CKRATES.  rsd_hist (optional, isor1 doubles) receives RSD/SPNORM after each species sweep.
+ * cell_sweeps (optional, n ints) receives the species sweeps each cell executed. */
+int uco_step(const uco_params *p, int li, int lj, double *rho, double *y, double *tk, double *ht,
+             double *stmf, double *stnd, const int32_t *iskip, double *qdot, double *qdsp,
+             uco_stats *st, double *rc_dump, const int64_t *dump_cells, int ndump,
+             double *rsd_hist, int32_t *cell_sweeps) {
+    return step_impl(p, li, lj, rho, y, tk, ht, stmf, stnd, iskip, qdot, qdsp, NULL, st, rc_dump, dump_cells,
+                     ndump, rsd_hist, cell_sweeps);
+}
+
+/* The same with the coupling inputs cp (uco_coupling; NULL: uco_step).  Returns -1 for the
+ * per-cell mode with coefficients, -2 for a flux without geometry. */
+int uco_step_cpl(const uco_params *p, int li, int lj, double *rho, double *y, double *tk, double *ht,
+                 double *stmf, double *stnd, const int32_t *iskip, double *qdot, double *qdsp,
+                 const uco_coupling *cp, uco_stats *st, double *rsd_hist, int32_t *cell_sweeps) {
+    return step_impl(p, li, lj, rho, y, tk, ht, stmf, stnd, iskip, qdot, qdsp, cp, st, NULL, NULL, 0, rsd_hist,
+                     cell_sweeps);
 }

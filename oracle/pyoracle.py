@@ -30,6 +30,13 @@ class Stats(C.Structure):
                 ("resdh", C.c_double), ("cell_sweeps", C.c_int64), ("newton_iters", C.c_int64)]
 
 
+class Coupling(C.Structure):
+    """uco_coupling (oracle/chem_oracle.c)"""
+    _fields_ = [(k, C.POINTER(C.c_double)) for k in
+                ("coef", "stc", "zc", "emu", "ekt", "tmu", "vdf", "ygeo", "xxc", "xxs", "yyc", "yys",
+                 "u", "v", "w")] + [("sigsp", C.c_double), ("isym", C.c_int32)]
+
+
 _lib = None
 
 
@@ -40,6 +47,7 @@ def lib():
             raise FileNotFoundError(f"{LIB} not built (make -C oracle oracle)")
         _lib = C.CDLL(str(LIB))
         _lib.uco_step.restype = C.c_int
+        _lib.uco_step_cpl.restype = C.c_int
         _lib.uco_rates_cell.restype = None
         _lib.uco_scales.restype = None
         _lib.uco_temperature.restype = C.c_double
@@ -72,10 +80,24 @@ def default_ref():
     return [sc.vref, sc.tref, sc.pref, sc.rtin, sc.rtot, sc.fo2in]
 
 
+def _coupling(cp, keep):
+    """ctypes view of a unicorn_amd.fields.Coupling (arrays kept alive in ``keep``)."""
+    s = Coupling()
+    for k, _ in Coupling._fields_[:-2]:
+        a = getattr(cp, k)
+        if a is not None:
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            setattr(s, k, ptr(a))
+    s.sigsp, s.isym = cp.sigsp, cp.isym
+    return s
+
+
 def step(field, isor1=100, isor2=100, relxsp=0.99, tolrsp=1e-8, relxh=0.99, tolrh=1e-8,
-         mode=MODE_GLOBAL, nthreads=None, dump_cells=(), ref=None) -> StepOut:
+         mode=MODE_GLOBAL, nthreads=None, dump_cells=(), ref=None, coupling=None) -> StepOut:
     """Advance ``field`` (a unicorn_amd.fields.ChemField, modified in place) by one
-    chemistry step with the C restatement."""
+    chemistry step with the C restatement; with ``coupling`` (a
+    unicorn_amd.fields.Coupling) the reference's coupled sweeps, serially."""
     p = Params()
     p.ref[:] = ref or default_ref()
     p.dt, p.relxsp, p.tolrsp, p.relxh, p.tolrh = field.dt, relxsp, tolrsp, relxh, tolrh
@@ -93,11 +115,21 @@ def step(field, isor1=100, isor2=100, relxsp=0.99, tolrsp=1e-8, relxh=0.99, tolr
     rc_dump = np.zeros((len(dump), 550)) if len(dump) else None
     hist = np.full(max(isor1, 1), np.nan)
     pcs = np.zeros(n, dtype=np.int32)
-    lib().uco_step(C.byref(p), field.li, field.lj, ptr(field.rho), ptr(field.y), ptr(field.tk),
-                   ptr(field.ht), ptr(field.stmf), ptr(field.stnd), ptr(iskip, C.c_int32),
-                   ptr(qdot), ptr(qdsp), C.byref(st), ptr(rc_dump),
-                   ptr(dump, C.c_int64) if len(dump) else None, len(dump), ptr(hist),
-                   ptr(pcs, C.c_int32))
+    if coupling is not None:
+        assert not len(dump), "RC dumps: chemistry-only steps"
+        keep = []
+        cs = _coupling(coupling, keep)
+        rc = lib().uco_step_cpl(C.byref(p), field.li, field.lj, ptr(field.rho), ptr(field.y), ptr(field.tk),
+                                ptr(field.ht), ptr(field.stmf), ptr(field.stnd), ptr(iskip, C.c_int32),
+                                ptr(qdot), ptr(qdsp), C.byref(cs), C.byref(st), ptr(hist), ptr(pcs, C.c_int32))
+        if rc:
+            raise ValueError(f"uco_step_cpl: {rc}")
+    else:
+        lib().uco_step(C.byref(p), field.li, field.lj, ptr(field.rho), ptr(field.y), ptr(field.tk),
+                       ptr(field.ht), ptr(field.stmf), ptr(field.stnd), ptr(iskip, C.c_int32),
+                       ptr(qdot), ptr(qdsp), C.byref(st), ptr(rc_dump),
+                       ptr(dump, C.c_int64) if len(dump) else None, len(dump), ptr(hist),
+                       ptr(pcs, C.c_int32))
     return StepOut(st.isor1, st.isor2, st.resdc, st.resdh, st.cell_sweeps, st.newton_iters,
                    qdot, qdsp, rc_dump, hist, pcs)
 

@@ -9,7 +9,9 @@
  * so the global sweep stops are those of the nj x ni field.
  *
  * One "chemistry step" is what the reference does around SPGSOLV for the
- * chemistry-only contract (all transport coefficients zero):
+ * chemistry-only contract (all transport coefficients zero), or, with
+ * FLAG_COUPLED, with the coefficients and transport inputs the input file holds
+ * (fixed over the steps) in place of SPGSOLV's assembly (spgsolv.f:233-1290):
  *   main:821-828           clip Y<0, N2 = 1 - sum(Y_1..51)        (all cells)
  *   spgsolv.f:98-232       SA=FA, FPZ=HT, SST=STMF/STND, EMU, EKT=0
  *   spgsolv.f:350-1290     E/W/P/Q, ZE..ZS, STE..STS = 0 (zero transport)
@@ -101,12 +103,18 @@ typedef struct {
     double *rho, *y, *tk, *ht, *u, *v, *w, *stmf, *stnd; /* field (nj,ni) j fastest; y[52][n] */
     int32_t *iskip;
     int64_t *dump_cells; /* field indices whose RC/SOUR are dumped after step 1 */
+    /* FLAG_COUPLED: coef[204][n] (E,W,P,Q of species 1..51 in /DUMMY/+/DUM0/ order),
+     * stc[4][n] (STE,STW,STN,STS), zc[4][n] (ZE,ZW,ZN,ZS), ekt[n], tmu[n], vdf[52][n],
+     * ygeo[n] (Y), xxc[ni], xxs[ni], yyc[nj], yys[nj], isym */
+    double *coef, *stc, *zc, *ekt, *tmu, *vdf, *ygeo, *xxc, *xxs, *yyc, *yys;
+    int64_t isym;
 } ucs_in;
 
 #define UCS_MAGIC 0x31534355 /* 'UCS1' */
 #define FLAG_FORCE_SWEEPS 1  /* tolrsp/tolrh ignored: exactly isor1/isor2 sweeps */
 #define FLAG_PROBE_FLUX 2    /* tests of the drop-in's contract: a non-zero VDF at one cell */
 #define FLAG_PROBE_COEF 4    /* tests of the drop-in's contract: a non-zero /DUMMY/ coefficient */
+#define FLAG_COUPLED 8       /* coefficients and transport inputs follow the dump cells */
 
 static void rd(void *p, size_t n, FILE *f) { if (fread(p, 1, n, f) != n) die("short read"); }
 static void wr(const void *p, size_t n, FILE *f) { if (fwrite(p, 1, n, f) != n) die("short write"); }
@@ -127,6 +135,17 @@ static void ucs_read(const char *path, ucs_in *in) {
     for (int q = 1; q < 8; ++q) { *fields[q] = malloc(8 * n); rd(*fields[q], 8 * n, f); }
     in->iskip = malloc(4 * n); rd(in->iskip, 4 * n, f);
     in->dump_cells = malloc(8 * (in->ndump + 1)); rd(in->dump_cells, 8 * in->ndump, f);
+    if (in->flags & FLAG_COUPLED) {
+        struct { double **p; size_t cnt; } blk[] = {
+            {&in->coef, 204 * n}, {&in->stc, 4 * n}, {&in->zc, 4 * n}, {&in->ekt, n}, {&in->tmu, n},
+            {&in->vdf, (size_t)LSP * n}, {&in->ygeo, n}, {&in->xxc, (size_t)in->ni}, {&in->xxs, (size_t)in->ni},
+            {&in->yyc, (size_t)in->nj}, {&in->yys, (size_t)in->nj}};
+        for (size_t b = 0; b < sizeof blk / sizeof blk[0]; ++b) {
+            *blk[b].p = malloc(8 * blk[b].cnt);
+            rd(*blk[b].p, 8 * blk[b].cnt, f);
+        }
+        rd(&in->isym, 8, f);
+    }
     fclose(f);
 }
 
@@ -144,15 +163,20 @@ static void clip_closure(void) { /* main:821-828 */
     }
 }
 
-static void density(void) { /* main:1056-1073 (RHO=RHONP via ISTDY=1 / main:1189) */
+/* main:1056-1073 (RHO=RHONP via ISTDY=1 / main:1189) with the embedded ni x nj field as
+ * the grid: its last I column is the one extrapolated */
+static void density(int ni, int nj) {
     static double rhonp[LE];
     for (int c = 0; c < LE; ++c) {
         double rbar = 0.0;
         for (int k = 0; k < LSP; ++k) rbar += cb04_.fsp[k][c] / cb02_.wm[k];
         rhonp[c] = S.pback / (cb06_.beta1 * cb04_.tk[c] * rbar);
     }
-    for (int c = LE - LJ; c < LE; ++c) rhonp[c] = 2.0 * rhonp[c - LJ] - rhonp[c - 2 * LJ];
-    for (int c = 0; c < LE; c += LJ) rhonp[c] = rhonp[c + 1];
+    for (int j = 0; j < nj; ++j) {
+        const int c = fidx(j, ni - 1);
+        rhonp[c] = 2.0 * rhonp[c - LJ] - rhonp[c - 2 * LJ];
+    }
+    for (int i = 0; i < ni; ++i) rhonp[fidx(0, i)] = rhonp[fidx(1, i)];
     memcpy(cb04_.rho, rhonp, sizeof rhonp);
 }
 
@@ -209,6 +233,19 @@ int main(int argc, char **argv) {
             soot_.stmf[c] = in.stmf[q]; soot_.stnd[c] = in.stnd[q];
         }
     cb03_.dt = in.dt;
+    if (in.flags & FLAG_COUPLED) {
+        cb01_.isym = (int32_t)in.isym;
+        for (int i = 0; i < ni; ++i) { cb03_.xxc[i] = in.xxc[i]; cb03_.xxs[i] = in.xxs[i]; }
+        for (int j = 0; j < nj; ++j) { cb03_.yyc[j] = in.yyc[j]; cb03_.yys[j] = in.yys[j]; }
+        for (int i = 0; i < ni; ++i)
+            for (int j = 0; j < nj; ++j) {
+                size_t q = (size_t)i * nj + j;
+                int c = fidx(j, i);
+                cb03_.y[c] = in.ygeo[q];
+                cb07_.tmu[c] = in.tmu[q];
+                for (int k = 0; k < LSP; ++k) cb07_.vdf[k][c] = in.vdf[(size_t)k * nj * ni + q];
+            }
+    }
 
     FILE *fo = fopen(argv[3], "wb");
     if (!fo) die("cannot open output");
@@ -231,6 +268,21 @@ int main(int argc, char **argv) {
         memset(&dumx_, 0, sizeof dumx_);
         memset(dums_.ste, 0, sizeof dums_.ste); memset(dums_.stw, 0, sizeof dums_.stw);
         memset(dums_.stn, 0, sizeof dums_.stn); memset(dums_.sts, 0, sizeof dums_.sts);
+        if (in.flags & FLAG_COUPLED) /* the assembled coefficients (spgsolv.f:233-1290) and EKT */
+            for (int i = 0; i < ni; ++i)
+                for (int j = 0; j < nj; ++j) {
+                    size_t q = (size_t)i * nj + j, nn = (size_t)nj * ni;
+                    int c = fidx(j, i);
+                    for (int g = 0; g < 204; ++g) {
+                        double v = in.coef[g * nn + q];
+                        if (g < 55) dummy_.a[g][c] = v; else dum0_.a[g - 55][c] = v;
+                    }
+                    dums_.ste[c] = in.stc[q]; dums_.stw[c] = in.stc[nn + q];
+                    dums_.stn[c] = in.stc[2 * nn + q]; dums_.sts[c] = in.stc[3 * nn + q];
+                    dumx_.ze[c] = in.zc[q]; dumx_.zw[c] = in.zc[nn + q];
+                    dumx_.zn[c] = in.zc[2 * nn + q]; dumx_.zs[c] = in.zc[3 * nn + q];
+                    dumy_.ekt[c] = in.ekt[q];
+                }
         if (in.flags & FLAG_PROBE_FLUX) cb07_.vdf[0][fidx(1, 1)] = 1.0e-5;
         if (in.flags & FLAG_PROBE_COEF) dummy_.a[0][fidx(1, 1)] = 1.0e-3;
         int32_t isor1 = (int32_t)in.isor1, isor2 = (int32_t)in.isor2;
@@ -271,7 +323,7 @@ int main(int argc, char **argv) {
             }
         resdh /= hnorm;
         resdc /= spnorm;
-        density();
+        density(ni, nj);
         int64_t counts[2] = {isor1, isor2};
         double res[5] = {resdc, resdh, t1 - t0, t2 - t1, t3 - t2};
         wr(counts, sizeof counts, fo);

@@ -23,6 +23,7 @@ LSP, LRX = 52, 544
 FLAG_FORCE_SWEEPS = 1
 FLAG_PROBE_FLUX = 2   # contract tests of the drop-in: a non-zero VDF at one cell
 FLAG_PROBE_COEF = 4   # contract tests of the drop-in: a non-zero /DUMMY/ coefficient
+FLAG_COUPLED = 8      # coefficients and transport inputs (unicorn_amd.fields.Coupling) follow
 
 
 def available(driver: str = "ref") -> bool:
@@ -52,22 +53,31 @@ class RefResult:
 
 
 def write_input(path, field, nsteps=1, isor1=100, isor2=100, relxsp=0.99, tolrsp=1e-8,
-                relxh=0.99, tolrh=1e-8, force=False, dump_cells=(), ref_values=None, probe=0):
+                relxh=0.99, tolrh=1e-8, force=False, dump_cells=(), ref_values=None, probe=0,
+                coupling=None):
     from unicorn_amd.scales import Scales
     sc = Scales()
     rv = ref_values or [sc.vref, sc.tref, sc.pref, sc.rtin, sc.rtot, sc.fo2in]
     dump_cells = np.asarray(dump_cells, dtype=np.int64)
     with open(path, "wb") as f:
         hdr = np.array([0x31534355, field.lj, field.li, nsteps, isor1, isor2,
-                        (FLAG_FORCE_SWEEPS if force else 0) | probe, len(dump_cells)], dtype="<i8")
+                        (FLAG_FORCE_SWEEPS if force else 0) | probe
+                        | (FLAG_COUPLED if coupling is not None else 0), len(dump_cells)], dtype="<i8")
         f.write(hdr.tobytes())
         f.write(np.asarray(rv, dtype="<f8").tobytes())
         f.write(np.array([field.dt, relxsp, tolrsp, relxh, tolrh], dtype="<f8").tobytes())
         z = np.zeros(field.n)
-        for a in (field.rho, field.y, field.tk, field.ht, z, z, z, field.stmf, field.stnd):
+        cp = coupling
+        u, v, w = (cp.u, cp.v, cp.w) if cp is not None else (z, z, z)
+        for a in (field.rho, field.y, field.tk, field.ht, u, v, w, field.stmf, field.stnd):
             f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
         f.write(np.ascontiguousarray(field.iskip, dtype="<i4").tobytes())
         f.write(dump_cells.tobytes())
+        if cp is not None:
+            assert cp.emu is None and abs(cp.sigsp - 0.9) < 1e-7, "the driver forms EMU and SIGSP as main does"
+            for a in (cp.coef, cp.stc, cp.zc, cp.ekt, cp.tmu, cp.vdf, cp.ygeo, cp.xxc, cp.xxs, cp.yyc, cp.yys):
+                f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+            f.write(np.array([cp.isym], dtype="<i8").tobytes())
 
 
 def read_output(path, field, nsteps, ndump) -> RefResult:

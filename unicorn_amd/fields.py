@@ -37,7 +37,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .scales import FUEL_INDEX, LSP, WSP, Scales, density, sensible_enthalpy
+from .scales import FUEL_INDEX, LSP, WSP, Scales, density, nasa7, sensible_enthalpy
 
 R_LEN = 0.02      # radial extent, m
 Z_LEN = 0.10      # axial extent, m
@@ -200,3 +200,105 @@ def batch_grids(ngrids: int = 256, seed: int = 2) -> list[ChemField]:
         dt = float(np.exp(rng.uniform(np.log(1e-6), np.log(1e-5))))
         out.append(flame_field(li, lj, fuel, seed=1000 + g, dt_seconds=dt))
     return out
+
+
+@dataclass
+class Coupling:
+    """What SPGSOLV hands the chemistry beyond the state (spgsolv.f:98-1290), in the
+    field layout: the hybrid-scheme coefficients of the species (E, W, P, Q of species
+    1..51 in the order of /DUMMY/ and /DUM0/, solvesp.f:24-58), of the soot scalars
+    (STE..STS) and of the enthalpy (ZE..ZS); the inputs of RDHSP (solveh.f:52-53: EMU,
+    VDF, TMU, EKT, SIGSP); the geometry of the RHSZ flux (solveh.f:113-115: XXC, XXS,
+    YYC, YYS, Y, ISYM); and the velocities of T(h) (solveh.f:1176-1177).  P couples
+    J+1, Q J-1, E I+1, W I-1."""
+    coef: np.ndarray     # (204, n)
+    stc: np.ndarray      # (4, n)  STE, STW, STN, STS
+    zc: np.ndarray       # (4, n)  ZE, ZW, ZN, ZS
+    ekt: np.ndarray      # (n,)
+    tmu: np.ndarray      # (n,)
+    vdf: np.ndarray      # (52, n)
+    ygeo: np.ndarray     # (n,)  radius Y(J,I)
+    xxc: np.ndarray      # (li,)
+    xxs: np.ndarray      # (li,)
+    yyc: np.ndarray      # (lj,)
+    yys: np.ndarray      # (lj,)
+    u: np.ndarray        # (n,)
+    v: np.ndarray        # (n,)
+    w: np.ndarray        # (n,)
+    emu: np.ndarray | None = None   # (n,); None: REI*BETA3*RHO as SPGSOLV sets it (spgsolv.f:175)
+    sigsp: float = float(np.float32(0.9))   # main:346
+    isym: int = 1
+
+    def zero_like(self) -> "Coupling":
+        """The same geometry and velocities with every coefficient and flux input zero."""
+        z = np.zeros_like
+        return Coupling(z(self.coef), z(self.stc), z(self.zc), z(self.ekt), z(self.tmu), z(self.vdf),
+                        self.ygeo, self.xxc, self.xxs, self.yyc, self.yys, self.u, self.v, self.w,
+                        None, self.sigsp, self.isym)
+
+
+def synthetic_coupling(f: ChemField, seed: int = 0, strength: float = 0.1, flux: float = 1e-3,
+                       velocity: float = 0.3, scales: Scales | None = None) -> Coupling:
+    """Seeded coupling inputs for a field, shaped like a diffusion-dominated hybrid
+    scheme: every face gets a conductance g >= 0 (uniform in [0.5, 1.5], times a
+    per-species factor in [0.5, 2]) and the two cells sharing it see -strength*rho*g
+    (E of cell i equals W of cell i+1), so each cell's equation stays diagonally
+    dominant (PPP = RHO - (E+W+P+Q) > RHO).  Soot faces are ten times weaker.  VDF,
+    TMU and EKT (positive, seeded) are scaled together so that the median |DT/2 x flux
+    divergence| is `flux` times the median |RHO*HT|.  Uniform spacing on the unit
+    square, radius Y = J*dy, ISYM = 1; velocities uniform in [-velocity, velocity]."""
+    sc = scales or Scales()
+    rng = np.random.Generator(np.random.PCG64(seed))
+    li, lj, n = f.li, f.lj, f.n
+    rho = f.rho.reshape(li, lj)
+
+    def faces(scale):
+        gx = rng.uniform(0.5, 1.5, (li + 1, lj)) * scale   # face between i-1 and i at index i
+        gy = rng.uniform(0.5, 1.5, (li, lj + 1)) * scale   # face between j-1 and j at index j
+        e = -strength * rho * gx[1:, :]
+        w = -strength * rho * gx[:-1, :]
+        p = -strength * rho * gy[:, 1:]
+        q = -strength * rho * gy[:, :-1]
+        return [a.ravel() for a in (e, w, p, q)]
+
+    coef = np.empty((204, n))
+    for m in range(51):
+        coef[4 * m:4 * m + 4] = faces(rng.uniform(0.5, 2.0))
+    stc = np.array(faces(0.1))
+    zc = np.array(faces(1.0))
+    dx, dy = 1.0 / (li - 1), 1.0 / (lj - 1)
+    xxc = np.full(li, dx); xxs = np.full(li, dx)
+    yyc = np.full(lj, dy); yys = np.full(lj, dy)
+    ygeo = np.tile(np.arange(lj) * dy, li)
+    u = rng.uniform(-velocity, velocity, n)
+    v = rng.uniform(-velocity, velocity, n)
+    w = np.zeros(n)
+    vdf = rng.uniform(0.5, 2.0, (LSP, n))
+    tmu = rng.uniform(0.0, 1.0, n)
+    ekt = rng.uniform(0.0, 1.0, n)
+    cp = Coupling(coef, stc, zc, ekt, tmu, vdf, ygeo, xxc, xxs, yyc, yys, u, v, w)
+    # calibrate the RDHSP inputs: the flux is linear in them (numpy, approximate)
+    pol = nasa7()
+    T = f.tk * sc.tstr
+    hi = T > 1000.0
+    hssp = np.empty((LSP, n))
+    for k in range(LSP):
+        a = np.where(hi[None, :], pol[k, 7:12, None], pol[k, 0:5, None])
+        hssp[k] = (a[0] * T + a[1] * T ** 2 / 2 + a[2] * T ** 3 / 3 + a[3] * T ** 4 / 4
+                   + a[4] * T ** 5 / 5) / sc.wm[k] * sc.gasc / sc.wmstr / sc.hstr
+    emu = sc.rei * sc.beta3 * f.rho
+    a0 = emu[None, :] * vdf + (sc.rei * tmu / cp.sigsp - 0.1 * ekt)[None, :]
+    rd = (a0 * hssp).reshape(LSP, li, lj)
+    y = f.y.reshape(LSP, li, lj)
+    div = (((rd[:, 2:, 1:-1] + rd[:, 1:-1, 1:-1]) * (y[:, 2:, 1:-1] - y[:, 1:-1, 1:-1])
+            - (rd[:, 1:-1, 1:-1] + rd[:, :-2, 1:-1]) * (y[:, 1:-1, 1:-1] - y[:, :-2, 1:-1])) / dx / dx
+           + ((rd[:, 1:-1, 2:] + rd[:, 1:-1, 1:-1]) * (y[:, 1:-1, 2:] - y[:, 1:-1, 1:-1])
+              - (rd[:, 1:-1, 1:-1] + rd[:, 1:-1, :-2]) * (y[:, 1:-1, 1:-1] - y[:, 1:-1, :-2])) / dy / dy)
+    x = np.abs(f.dt * 0.5 * div.sum(axis=0))
+    target = flux * np.median(np.abs(f.rho * f.ht))
+    med = float(np.median(x))
+    s = target / med if med > 0 else 1.0
+    cp.vdf = vdf * s
+    cp.tmu = tmu * s
+    cp.ekt = 0.1 * ekt * s
+    return cp
