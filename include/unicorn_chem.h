@@ -17,10 +17,10 @@
  *    built for the grid PARAMETER(LI,LJ) of the Fortran build, and read and
  *    write the COMMON blocks /CB01/ /CB02/ /CB03/ /CB04/ /CB06/ /CB08/ /REAC/
  *    /HFORM/ /HRLS/ /DUMD/ /SOOT/ /DUMY/ /DUMS/ /DUMR/ /DUMMY/ /DUM0/ /DUMX/
- *    exactly as the reference routines do.  Only the chemistry-only contract
- *    (all transport coefficients E/W/P/Q, ZE..ZS, STE..STS zero) is executed
- *    on the GPU; a call with non-zero coefficients stops the program with a
- *    message (there is no CPU fallback).
+ *    exactly as the reference routines do, with or without the transport
+ *    coefficients SPGSOLV assembles (E/W/P/Q, ZE..ZS, STE..STS, the RDHSP inputs)
+ *    and with the velocities of T(h); everything runs on the GPU (there is no CPU
+ *    fallback).
  *
  * 2. The device API (libunicorn_chem.so), used by the shim and directly by
  *    host programs: a context holds the scales and scratch, a step advances a
@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define UC_ABI_VERSION 1
+#define UC_ABI_VERSION 2
 
 enum {
     UC_OK = 0,
@@ -112,6 +112,25 @@ typedef struct {
     int32_t graph;      /* 1: each sweep segment is replayed from a captured hipGraph */
 } uc_tuning;
 
+/* What SPGSOLV hands the chemistry beyond the state (spgsolv.f:98-1290); device pointers
+ * in the field layout, each may be NULL (zero).  coef[4m+f][n] is face f (0 E: I+1,
+ * 1 W: I-1, 2 P: J+1, 3 Q: J-1) of species m+1, in the order of /DUMMY/ then /DUM0/
+ * (solvesp.f:24-58: EA0, WA0, PA0, QA0, EA1, ...).  The flux inputs (emu, ekt, tmu,
+ * vdf) need the geometry; emu NULL is REI*BETA3*RHO as SPGSOLV sets it (spgsolv.f:175). */
+typedef struct {
+    const double *coef;          /* [204][n] species coefficients (solvesp.f:424-426) */
+    const double *stc;           /* [4][n] STE, STW, STN, STS (solvesp.f:2812-2825) */
+    const double *zc;            /* [4][n] ZE, ZW, ZN, ZS (solveh.f:1154-1158) */
+    const double *emu, *ekt, *tmu;  /* [n] RDHSP inputs (solveh.f:52-53) */
+    const double *vdf;           /* [52][n] */
+    const double *ygeo;          /* [n] /CB03/ Y (radius) */
+    const double *xxc, *xxs;     /* [li] /CB03/ XXC, XXS */
+    const double *yyc, *yys;     /* [lj] /CB03/ YYC, YYS */
+    const double *u, *v, *w;     /* [n] velocities: ENTH = (FPZ-(U*U+V*V+W*W)/2)*HSTR+... */
+    double sigsp;                /* SOLVEH's SIGSP (main:346: 0.9) */
+    int32_t isym;                /* /CB01/ ISYM */
+} uc_coupling;
+
 typedef struct uc_context uc_context;
 
 int uc_abi_version(void);
@@ -142,6 +161,14 @@ int uc_destroy(uc_context *ctx);
  * calls uc_cells_changed. */
 int uc_step(uc_context *ctx, const uc_field *field, const uc_controls *ctl, void *stream,
             uc_stats *stats);
+
+/* The same step with the coupling inputs (uc_coupling; NULL: uc_step).  Non-zero
+ * coefficients run the reference's lexicographic Gauss-Seidel (J outer, I inner; every
+ * cell reads its neighbours' latest iterates, solvesp.f:424-426, solveh.f:1154-1160) as a
+ * pipelined anti-diagonal wavefront, exactly; the flux inputs add the species-diffusion
+ * enthalpy flux to RHSZ (solveh.f:59-115).  Global stops only (UC_MODE_GLOBAL); one grid. */
+int uc_step_coupled(uc_context *ctx, const uc_field *field, const uc_controls *ctl,
+                    const uc_coupling *coupling, void *stream, uc_stats *stats);
 
 /* Same, on host arrays (copies in and out; the PCIe cost is in ms_total). */
 int uc_step_host(uc_context *ctx, const uc_field *host_field, const uc_controls *ctl,

@@ -6,7 +6,8 @@ against that library instead of the flang-built routines, so these tests run
 the shipped Fortran ABI end to end: COMMON layout, argument lists, ISOR1/ISOR2
 in/out, the SPGSOLV epilogue and density update left to the caller.  On the GPU
 its outputs must match the reference's own (the golden fixtures) to the same
-bar as the device API.
+bar as the device API, with and without the transport coefficients, enthalpy flux
+inputs and velocities SPGSOLV leaves in the COMMON blocks.
 """
 from __future__ import annotations
 
@@ -16,7 +17,8 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from golden_io import CASES, assert_rates_close, assert_state_close, expected, load
+from golden_io import (CASES, COUPLED_CASES, assert_rates_close, assert_state_close, expected, load,
+                       load_coupling)
 from oracle import refrun
 from unicorn_amd.fields import ChemField
 
@@ -55,19 +57,48 @@ def test_shim_without_gpu_stops_with_a_message():
     assert "unicorn_fshim: ckrates_:" in e.value.stderr
 
 
-@pytest.mark.parametrize("probe,message", [(refrun.FLAG_PROBE_FLUX, "species-diffusion enthalpy flux"),
-                                           (refrun.FLAG_PROBE_COEF, "non-zero transport coefficient in /DUMMY/")])
-def test_shim_refuses_transport_before_the_gpu(probe, message):
-    """The drop-in implements the chemistry-only contract: a non-zero VDF (with SPGSOLV's
-    non-zero EMU) or a non-zero /DUMMY/ coefficient at one interior cell stops the
-    program with a message, before any GPU call, GPU or not."""
+def _compare(r, f0, e, dump, label):
+    inter = f0.interior()
+    assert list(r.isor1) == list(e["isor1"]), label
+    assert list(r.isor2) == list(e["isor2"]), label
+    assert np.allclose(r.resdc, e["resdc"], rtol=1e-9, atol=0)
+    assert np.allclose(r.resdh, e["resdh"], rtol=1e-9, atol=0)
+    got = ChemField(f0.li, f0.lj, r.rho, r.y, r.tk, r.ht, r.stmf, r.stnd, f0.iskip, f0.dt, f0.fuel)
+    assert_state_close(got, e, inter, label)
+    assert_rates_close(r.qdot, r.qdsp[:, dump], e["qdot"], e["qdsp_dump"], inter, label)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("probe", [refrun.FLAG_PROBE_FLUX, refrun.FLAG_PROBE_COEF])
+def test_shim_single_coefficient_probes_match_the_reference(probe):
+    """One non-zero VDF (with SPGSOLV's EMU: an enthalpy flux around one cell) or one
+    non-zero /DUMMY/ coefficient: the drop-in takes the coupled path and matches the
+    flang-built routines run on the same COMMON blocks."""
+    if not (refrun.available("shim") and refrun.available("ref")):
+        pytest.skip("reference and shim drivers not built")
+    f0, z = load("sooty")
+    dump = z["dump_cells"]
+    ref = refrun.run(f0.copy(), nsteps=1, probe=probe)
+    e = dict(isor1=ref.isor1, isor2=ref.isor2, resdc=ref.resdc, resdh=ref.resdh, rho=ref.rho, y=ref.y,
+             tk=ref.tk, ht=ref.ht, stmf=ref.stmf, stnd=ref.stnd, qdot=ref.qdot, qdsp_dump=ref.qdsp[:, dump])
+    print(f"shim probe {probe}", flush=True)
+    r = refrun.run(f0.copy(), nsteps=1, driver="shim", probe=probe)
+    _compare(r, f0, e, dump, f"probe {probe}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", COUPLED_CASES)
+def test_shim_matches_coupled_fixture(name):
+    """The coupled fixture (every coefficient, the flux inputs, geometry and velocities
+    in the COMMON blocks) through the drop-in, against the reference's outputs."""
     if not refrun.available("shim"):
         pytest.skip("ref_driver_shim not built")
-    f0, _ = load("sooty")
-    with pytest.raises(subprocess.CalledProcessError) as e:
-        refrun.run(f0, nsteps=1, driver="shim", probe=probe)
-    assert e.value.returncode == 3
-    assert message in e.value.stderr, e.value.stderr
+    f0, z = load(name)
+    cp = load_coupling(z)
+    for ns in [int(s) for s in z["steps"]]:
+        print(f"shim coupled {name} steps {ns}", flush=True)
+        r = refrun.run(f0.copy(), nsteps=ns, driver="shim", coupling=cp)
+        _compare(r, f0, expected(z, ns), z["dump_cells"], f"shim {name}/{ns}")
 
 
 @pytest.mark.gpu

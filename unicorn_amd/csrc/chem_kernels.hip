@@ -46,7 +46,7 @@ constexpr int WAVE = 64;
 constexpr int64_t UC_MAX_CELLS = 4000000;  // 54 doubles x cells < 2^31 bytes
 
 struct UcParams {
-  double tstr, alstr, vstr, rstr, wmstr, hstr, gasc, pback, beta1;
+  double tstr, alstr, vstr, rstr, wmstr, hstr, gasc, pback, beta1, rei, beta3;
   double wm[NSP], rwm[NSP], hfo[NSP];
   double nasa[NSP][14];
   double rcfac[NRX];  // C_m / prod W of the slot's reactants (ckrates.f:83-85, :796-1333)
@@ -81,11 +81,12 @@ __device__ __forceinline__ uint64_t wave_uniform(uint64_t v) {
 
 constexpr int SLAB_SLOTS = 664;  // = UC_SLAB_SLOTS of chem_gen.inc
 
-__device__ __forceinline__ RcSlab tile_slab(double* slabs, int64_t tile) {
-  double* t = (double*)wave_uniform((uint64_t)(slabs + tile * (int64_t)SLAB_SLOTS * 64));
-  return RcSlab{__builtin_amdgcn_make_buffer_rsrc(t, 0, SLAB_SLOTS * 512, 0x00020000),
-                (threadIdx.x & 63u) * 8u};
+__device__ __forceinline__ RcSlab tile_rsrc(const double* base, int64_t tile, int slots) {
+  double* t = (double*)wave_uniform((uint64_t)(base + tile * (int64_t)slots * 64));
+  return RcSlab{__builtin_amdgcn_make_buffer_rsrc(t, 0, slots * 512, 0x00020000), (threadIdx.x & 63u) * 8u};
 }
+
+__device__ __forceinline__ RcSlab tile_slab(double* slabs, int64_t tile) { return tile_rsrc(slabs, tile, SLAB_SLOTS); }
 
 // A field array ([k][n] species-major, or [n]) of one cell: the cell's byte
 // offset is the only per-lane part; k*n*8 goes in the scalar offset.
@@ -106,6 +107,30 @@ __device__ __forceinline__ FieldBuf field_buf(const double* base, int64_t bytes,
 
 #include "chem_gen.inc"
 static_assert(UC_SLAB_SLOTS == SLAB_SLOTS, "slab layout");
+
+// Reads of the coupled sweep (uc_sweep_cpl): slab slots, the cell's coupling slots
+// (UC_CPL + s, a tile of UC_CPL_SLOTS per wave like the slab), and the neighbours'
+// current iterates (UC_NBR + 4*row + dir, dir 0: J+1, 1: J-1, 2: I+1, 3: I-1) from the
+// field-layout iterate array gs [54][n].  Slot numbers are compile-time constants, so
+// each ld() inlines to one buffer load.
+struct RcCpl {
+  RcSlab slab, cpl;
+  __amdgpu_buffer_rsrc_t g;
+  uint32_t nb8[4];  // byte offsets of the four neighbour cells
+  int32_t n8;       // row stride of gs in bytes
+  __device__ __forceinline__ double ld(int s) const {
+    if (s < UC_CPL) return slab.ld(s);
+    if (s < UC_NBR) return cpl.ld(s - UC_CPL);
+    const int r = (s - UC_NBR) >> 2, d = (s - UC_NBR) & 3;
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(g, nb8[d], r * n8, 0));
+  }
+  __device__ __forceinline__ void st(int s, double v) const { slab.st(s, v); }
+  __device__ __forceinline__ void fence() {
+    slab.fence();
+    cpl.fence();
+    asm volatile("" : "+s"(g));
+  }
+};
 
 thread_local std::string g_err;
 
@@ -355,6 +380,7 @@ struct SourcesArgs {
   double* sst;          // [2][npad]  SST1, SST2
   double* rhsz;         // [npad]
   double* qdot; double* qdsp;  // packed field arrays (solveh.f:1137-1142)
+  const double* flux;   // [npad] DT/2 x species-diffusion enthalpy flux divergence (null: zero)
 };
 
 // Iterates at the decided count, SST2 completion, N2 closure (solvesp.f:2839-2914)
@@ -434,7 +460,7 @@ __global__ void __launch_bounds__(WAVE) k_sources(const UcParams* __restrict__ P
   const double crad = P->alstr / P->rstr / P->vstr / P->vstr / P->vstr;
   const double qrad = crad * (double)0.4f * (4.0 * 5.669e-08 * plankc * (pow(T, 4.0) - 8.1e+09)
                                             + 3.334e-04 * stmf * (rho / 1.9e+03) * T5);
-  double rhsz = -qrad * dt;
+  double rhsz = a.flux ? a.flux[q] - qrad * dt : -qrad * dt;  // solveh.f:113-115
   const double rconv = P->vstr / P->wmstr / P->alstr * 1.0e-06;
   double prod = 0.0, hrr = 0.0;
   const FieldBuf QD = field_buf(a.qdsp, n * 8 * NSP, c);
@@ -508,6 +534,8 @@ struct FinishArgs {
   double* res_part;  // [2][ntiles] per-tile RESDC, RESDH partial sums
   int32_t ntiles;
   double* fpz_out;   // null: SPGSOLV epilogue + density; else FPZ here and TK only
+  const double* fpz_in;  // [npad] relaxed FPZ of the coupled relaxation (null: relax here)
+  const double* u; const double* v; const double* w;  // [n] velocities of T(h) (null: zero)
 };
 
 __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg, FinishArgs a) {
@@ -527,16 +555,21 @@ __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg,
     const double rho = a.rho[c], ht = a.ht[c];
     // enthalpy relaxation (solveh.f:1147-1167), zero transport: PPP = RHO
     double fpz = ht;
-    const double inc = a.rhsz[q] * G.zsor1 / rho;
-    const bool percell = G.mode == UC_MODE_PERCELL;
-    for (int32_t sw = 0; sw < G.nh; ++sw) {
-      const double fnew = G.zsor2 * fpz + inc;
-      const double d = fabs(fpz - fnew);
-      fpz = fnew;
-      if (percell && d <= G.tolrh) break;
+    if (a.fpz_in) {
+      fpz = a.fpz_in[q];
+    } else {
+      const double inc = a.rhsz[q] * G.zsor1 / rho;
+      const bool percell = G.mode == UC_MODE_PERCELL;
+      for (int32_t sw = 0; sw < G.nh; ++sw) {
+        const double fnew = G.zsor2 * fpz + inc;
+        const double d = fabs(fpz - fnew);
+        fpz = fnew;
+        if (percell && d <= G.tolrh) break;
+      }
     }
-    // T from h (solveh.f:1170-1230)
-    double enth = fpz * P->hstr;
+    // T from h (solveh.f:1170-1230); ENTH = (FPZ - (U*U+V*V+W*W)/2)*HSTR + sum Y*HFO*HSTR
+    const double uc = a.u ? a.u[c] : 0.0, vc = a.v ? a.v[c] : 0.0, wc = a.w ? a.w[c] : 0.0;
+    double enth = (fpz - 0.5 * (uc * uc + vc * vc + wc * wc)) * P->hstr;
 #pragma unroll
     for (int k = 0; k < NSP; ++k) enth += s[k] * P->hfo[k] * P->hstr;
     bool hiT = false;
@@ -636,6 +669,222 @@ __global__ void k_zero(double* __restrict__ p, int64_t n) {
   if (i < n) p[i] = 0.0;
 }
 
+// ---- the coupled step (transport coefficients non-zero): the reference's lexicographic
+// Gauss-Seidel (J outer, I inner) as a pipelined anti-diagonal wavefront.  A cell
+// (i, j) of sweep s reads (i-1, j) and (i, j-1) of sweep s and (i+1, j), (i, j+1) of
+// sweep s-1, so wavefront step t updates diagonal d = t - 2s (d = i + j - 2) of every
+// sweep s of the segment at once: the diagonals of one step are never adjacent, every
+// value a cell reads was written in an earlier step, and the result is the
+// reference's sweep order exactly.  The cell list is in diagonal order (j ascending);
+// dfirst[d] .. dfirst[d+1]-1 are the list positions of diagonal d.
+
+// SA = FA, SST1 = STMF, SST2 = STND and FPZ = HT at every cell (spgsolv.f:169-231)
+__global__ void k_cpl_init(const double* __restrict__ y, const double* __restrict__ stmf,
+                           const double* __restrict__ stnd, const double* __restrict__ ht, int64_t n,
+                           double* __restrict__ gs, double* __restrict__ gz) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  for (int k = 0; k < NSP; ++k) gs[k * n + c] = y[k * n + c];
+  gs[52 * n + c] = stmf[c];
+  gs[53 * n + c] = stnd[c];
+  gz[c] = ht[c];
+}
+
+// The cell's coupling tile: per group g (species 1..51, the soot scalars, the enthalpy)
+// PPP = RHO-(E+W+P+Q) (solvesp.f:424, :2812; solveh.f:1154) and P, Q, E, W.  coef:
+// [204][n] E, W, P, Q per species (/DUMMY/ + /DUM0/ order); stc, zc: [4][n] E, W, N(P),
+// S(Q).  A null array is zero.
+__global__ void k_cpl_coef(const double* __restrict__ rho, const int32_t* __restrict__ cells, int64_t npad,
+                           int64_t n, const double* __restrict__ coef, const double* __restrict__ stc,
+                           const double* __restrict__ zc, double* __restrict__ cplc) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npad) return;
+  bool valid;
+  const int64_t c = list_cell(cells, q, valid);
+  double* t = cplc + (q / WAVE) * (int64_t)UC_CPL_SLOTS * WAVE + (q & (WAVE - 1));
+  const double r = rho[c];
+  for (int g = 0; g < 53; ++g) {
+    const double* a = g < 51 ? (coef ? coef + (int64_t)4 * g * n : nullptr) : (g == 51 ? stc : zc);
+    const double e = a ? a[c] : 0.0, w = a ? a[n + c] : 0.0, p = a ? a[2 * n + c] : 0.0,
+                 qq = a ? a[3 * n + c] : 0.0;
+    t[(5 * g + 0) * WAVE] = r - (e + w + p + qq);
+    t[(5 * g + 1) * WAVE] = p;
+    t[(5 * g + 2) * WAVE] = qq;
+    t[(5 * g + 3) * WAVE] = e;
+    t[(5 * g + 4) * WAVE] = w;
+  }
+}
+
+struct WaveArgs {
+  const double* rho; const double* stmf; const double* stnd;
+  const int32_t* cells; const int32_t* dfirst; const GridTab* gt;
+  int64_t npad, n;
+  int32_t lj, ndiag, maxtiles, ntiles;
+  int32_t t;          // wavefront step
+  int32_t S;          // sweeps of the segment
+  double* slabs; const double* cplc;
+  double* gs;         // [54][n] species iterates, SST1, SST2 (field layout)
+  double* snap;       // [S][54][npad] iterates after each sweep of the segment
+  double* rsd_part;   // rows start+s, accumulated per tile (a tile may span diagonals)
+  // enthalpy (k_hsweep_cpl)
+  const double* rhsz; double* gz; double* zsnap;  // [npad], [n], [S][npad]
+};
+
+// Block (si, ti): tile ti of diagonal t - 2*si in sweep start+si; lanes outside the
+// diagonal compute but write nothing.
+__device__ __forceinline__ bool wave_cell(const WaveArgs& a, int32_t& si, int64_t& tile, int64_t& q,
+                                          bool& mine, int64_t& c) {
+  si = (int32_t)blockIdx.x / a.maxtiles;
+  const int32_t ti = (int32_t)blockIdx.x % a.maxtiles;
+  const int32_t d = a.t - 2 * si;
+  if (si >= a.S || d < 0 || d >= a.ndiag) return false;
+  const int32_t q0 = a.dfirst[d], q1 = a.dfirst[d + 1];
+  tile = q0 / WAVE + ti;
+  if (tile * WAVE >= q1) return false;
+  q = tile * WAVE + (threadIdx.x & (WAVE - 1));
+  bool valid;
+  c = list_cell(a.cells, q, valid);
+  mine = valid && q >= q0 && q < q1;
+  return true;
+}
+
+__global__ void __launch_bounds__(WAVE) k_sweep_cpl(const UcParams* __restrict__ Pg, WaveArgs a) {
+  extern __shared__ double uc_lds[];
+  double* const lds = uc_lds + threadIdx.x;
+  UcP P = (UcP)Pg;
+  int32_t si;
+  int64_t tile, q, c;
+  bool mine;
+  if (!wave_cell(a, si, tile, q, mine, c)) return;
+  const GridTab& G = a.gt[0];
+  const int64_t n = a.n;
+  const FieldBuf GS = field_buf(a.gs, n * 8 * 54, c);
+  double s[54];
+#pragma unroll
+  for (int k = 0; k < 54; ++k) s[k] = GS.ld(k * n);
+  const int64_t lj = a.lj;
+  RcCpl rc{tile_slab(a.slabs, tile), tile_rsrc(a.cplc, tile, UC_CPL_SLOTS),
+           __builtin_amdgcn_make_buffer_rsrc((void*)a.gs, 0, (int)(n * 8 * 54), 0x00020000),
+           {(uint32_t)((c + 1) * 8), (uint32_t)((c - 1) * 8), (uint32_t)((c + lj) * 8), (uint32_t)((c - lj) * 8)},
+           (int32_t)(n * 8)};
+  double d2;
+  const double r = uc_sweep_cpl(P, rc, lds, s, a.rho[c], a.stmf[c], a.stnd[c], G.dt, G.sor1, G.sor2, &d2);
+  if (mine) {
+#pragma unroll
+    for (int k = 0; k < 54; ++k) GS.st(k * n, s[k]);
+    const FieldBuf SN = field_buf(a.snap + (int64_t)si * 54 * a.npad, a.npad * 8 * 54, q);
+#pragma unroll
+    for (int k = 0; k < 54; ++k) SN.st(k * a.npad, s[k]);
+  }
+  const double ws = wave_sum(mine ? r : 0.0);
+  if ((threadIdx.x & (WAVE - 1)) == 0) a.rsd_part[(int64_t)(G.start + si) * a.ntiles + tile] += ws;
+}
+
+// enthalpy: FNEW = SOR2*FPZ + (RHSZ-ZN*FPZ(J+1)-ZS*FPZ(J-1)-ZE*FPZ(I+1)-ZW*FPZ(I-1))*SOR1/PPP
+// (solveh.f:1154-1160); rows of rsd_part are the segment's sweeps (hfrom based)
+__global__ void __launch_bounds__(WAVE) k_hsweep_cpl(WaveArgs a) {
+  int32_t si;
+  int64_t tile, q, c;
+  bool mine;
+  if (!wave_cell(a, si, tile, q, mine, c)) return;
+  const GridTab& G = a.gt[0];
+  double dz = 0.0;
+  if (mine) {
+    const double* t = a.cplc + tile * (int64_t)UC_CPL_SLOTS * WAVE + (q & (WAVE - 1));
+    const int64_t lj = a.lj;
+    const double ppp = t[260 * WAVE], zn = t[261 * WAVE], zs = t[262 * WAVE], ze = t[263 * WAVE],
+                 zw = t[264 * WAVE];
+    const double fpz = a.gz[c];
+    double num = a.rhsz[q];
+    num = num - zn * a.gz[c + 1];
+    num = num - zs * a.gz[c - 1];
+    num = num - ze * a.gz[c + lj];
+    num = num - zw * a.gz[c - lj];
+    const double fnew = G.zsor2 * fpz + num * G.zsor1 / ppp;
+    dz = fabs(fpz - fnew);
+    a.gz[c] = fnew;
+    a.zsnap[(int64_t)si * a.npad + q] = fnew;
+  }
+  const double ws = wave_sum(dz);
+  if ((threadIdx.x & (WAVE - 1)) == 0) a.rsd_part[(int64_t)si * a.ntiles + tile] += ws;
+}
+
+// N2 closure (solvesp.f:2839-2914) of the iterates at the decided count, into gs: SSP
+// of every cell for the enthalpy flux (active cells closed, the others as entered)
+__global__ void k_cpl_close(const double* __restrict__ src, const int32_t* __restrict__ cells, int64_t npad,
+                            int64_t n, double* __restrict__ gs) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npad) return;
+  bool valid;
+  const int64_t c = list_cell(cells, q, valid);
+  if (!valid) return;
+  double s[54];
+  for (int k = 0; k < 54; ++k) s[k] = src[k * npad + q];
+  double ftot = 0.0;
+  for (int k = 0; k < 51; ++k) ftot += s[k];
+  ftot += s[52];
+  s[51] = 1.0 - ftot;
+  if (ftot > 1.0) {
+    for (int k = 0; k < 51; ++k) s[k] = s[k] / ftot;
+    s[52] = s[52] / ftot;
+    s[51] = 0.0;
+  }
+  for (int k = 0; k < 54; ++k) gs[k * n + c] = s[k];
+}
+
+// RDHSP = (EMU*VDF + REI*TMU/SIGSP - EKT)*HSSP at every cell (solveh.f:34-57); null
+// inputs are zero, a null emu is REI*BETA3*RHO (spgsolv.f:175)
+__global__ void k_cpl_rdhsp(const UcParams* __restrict__ Pg, const double* __restrict__ tk,
+                            const double* __restrict__ rho, const double* __restrict__ emu,
+                            const double* __restrict__ vdf, const double* __restrict__ tmu,
+                            const double* __restrict__ ekt, double sigsp, int64_t n, double* __restrict__ rd) {
+  UcP P = (UcP)Pg;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double T = tk[c] * P->tstr;
+  const double T2 = T * T, T3 = T * T2, T4 = T * T3, T5 = T * T4;
+  const bool hiT = T > 1000.0;
+  const double hconvr = P->gasc / P->wmstr / P->hstr;
+  const double e = emu ? emu[c] : P->rei * P->beta3 * rho[c];
+  const double tm = tmu ? tmu[c] : 0.0, ek = ekt ? ekt[c] : 0.0;
+  for (int k = 0; k < NSP; ++k) {
+    double pk[5];
+    for (int i = 0; i < 5; ++i) pk[i] = hiT ? P->nasa[k][7 + i] : P->nasa[k][i];
+    const double hssp = (pk[0] * T + pk[1] * T2 / 2.0 + pk[2] * T3 / 3.0 + pk[3] * T4 / 4.0 + pk[4] * T5 / 5.0)
+                        * P->rwm[k] * hconvr;
+    const double v = vdf ? vdf[k * n + c] : 0.0;
+    rd[k * n + c] = (e * v + P->rei * tm / sigsp - ek) * hssp;
+  }
+}
+
+// DT*0.5 times the species-diffusion enthalpy flux divergence (solveh.f:63-115) of every
+// active cell, from RDHSP and the closed species (gs) of the cell and its neighbours
+__global__ void k_cpl_flux(const double* __restrict__ rd, const double* __restrict__ gs,
+                           const int32_t* __restrict__ cells, int64_t npad, int64_t n, int32_t lj,
+                           const double* __restrict__ xxc, const double* __restrict__ xxs,
+                           const double* __restrict__ yyc, const double* __restrict__ yys,
+                           const double* __restrict__ ygeo, double sym, double dt, double* __restrict__ flux) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npad) return;
+  bool valid;
+  const int64_t ia = list_cell(cells, q, valid);
+  if (!valid) return;
+  const int64_t i = ia / lj, j = ia % lj;
+  double rdhe = 0.0, rdhw = 0.0, rdhn = 0.0, rdhs = 0.0, rdhp = 0.0;
+  for (int k = 0; k < NSP; ++k) {
+    const double* r = rd + k * n;
+    const double* y = gs + k * n;
+    rdhe = rdhe + (r[ia + lj] + r[ia]) * (y[ia + lj] - y[ia]);
+    rdhw = rdhw + (r[ia] + r[ia - lj]) * (y[ia] - y[ia - lj]);
+    rdhn = rdhn + (r[ia + 1] + r[ia]) * (y[ia + 1] - y[ia]);
+    rdhs = rdhs + (r[ia] + r[ia - 1]) * (y[ia] - y[ia - 1]);
+    rdhp = rdhp + r[ia] * (y[ia + 1] - y[ia - 1]);
+  }
+  const double x = (rdhe / xxs[i + 1] - rdhw / xxs[i]) / xxc[i] + (rdhn / yys[j + 1] - rdhs / yys[j]) / yyc[j]
+                   + sym * rdhp / ygeo[ia] / yyc[j];
+  flux[q] = dt * 0.5 * x;
+}
+
 // ---- batches: per-grid field arrays <-> the packed arrays (grid g at offset off_g)
 struct PackPtrs {
   double *rho, *y, *tk, *ht, *stmf, *stnd, *qdot, *qdsp;
@@ -732,6 +981,12 @@ struct StepRun {
   hipStream_t s = nullptr;
   int32_t ngrids = 0;
   bool clip = false;
+  // coupled step (one grid): coupling inputs, which parts run as wavefronts, and the
+  // relaxed FPZ the coupled enthalpy relaxation leaves for k_finish
+  uc_coupling cp{};
+  bool cpl = false, cpl_sp = false, cpl_h = false, cpl_flux = false, vel = false;
+  const double* fpz_in = nullptr;
+  int64_t cpl_sweeps = 0;
   float ms_rates = 0.f, ms_sweeps = 0.f, ms_sources = 0.f;
   int32_t sweep_launches = 0;
   std::vector<uc_field> fields;       // the caller's grids
@@ -750,6 +1005,8 @@ struct Layout {
   int32_t nbase = 0;             // active cells
   bool packed = false;           // batch: grids copied to the context's packed arrays
   bool have_prev = false;        // prev_nsw holds the previous step's sweeps
+  bool diag = false;             // coupled step: cells in anti-diagonal order (ctx->cpl_dfirst)
+  int32_t ndiag = 0, maxtiles = 0;  // diagonals; most tiles any diagonal's positions touch
 };
 
 // tuning of a step: the autotuner's choices (uc_tuning)
@@ -823,6 +1080,17 @@ struct uc_context {
   SegGraph seg;
   hipStream_t gs = nullptr;  // the context's stream for captured sweep segments
   StepRun run;
+  // coupled step (allocated at its first use)
+  double* cpl_it = nullptr;     // [54][max_cells] iterates SA*, SST1, SST2 in field layout
+  double* cpl_fpz = nullptr;    // [max_cells] FPZ in field layout
+  double* cpl_rd = nullptr;     // [52][max_cells] RDHSP
+  double* cpl_coef = nullptr;   // [max_pad/64][UC_CPL_SLOTS][64] PPP, P, Q, E, W per group
+  double* cpl_flux = nullptr;   // [max_pad]
+  double* cpl_snap = nullptr;   // [NSNAP_CPL][54][max_pad]
+  double* cpl_zsnap = nullptr;  // [NSNAP_CPL][max_pad]
+  int32_t* cpl_dfirst = nullptr;  // [ndiag+1] first list position of each diagonal
+  int64_t cpl_dfirst_cap = 0;
+  int32_t last_nh = 0;          // previous coupled step's enthalpy sweep count
 };
 
 namespace {
@@ -921,6 +1189,8 @@ int uc_create_batch(const uc_scales* scales, int64_t max_cells, int32_t max_grid
                                UC_SWEEP_LDS * 512));
     HIPCHK(hipFuncSetAttribute((const void*)k_sweep<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                UC_SWEEP_LDS * 512));
+    HIPCHK(hipFuncSetAttribute((const void*)k_sweep_cpl, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               UC_SWEEP_LDS * 512));
   }
   if (UC_PRD_LDS * 512 > 65536)
     HIPCHK(hipFuncSetAttribute((const void*)k_sources, hipFuncAttributeMaxDynamicSharedMemorySize, UC_PRD_LDS * 512));
@@ -945,7 +1215,9 @@ int uc_destroy(uc_context* ctx) {
                   (void*)ctx->rsd_part, (void*)ctx->res_part, (void*)ctx->rows, (void*)ctx->counters,
                   (void*)ctx->pk.rho, (void*)ctx->pk.y, (void*)ctx->pk.tk, (void*)ctx->pk.ht, (void*)ctx->pk.stmf,
                   (void*)ctx->pk.stnd, (void*)ctx->pk.qdot, (void*)ctx->pk.qdsp, (void*)ctx->pk_iskip,
-                  (void*)ctx->nsel})
+                  (void*)ctx->nsel, (void*)ctx->cpl_it, (void*)ctx->cpl_fpz, (void*)ctx->cpl_rd,
+                  (void*)ctx->cpl_coef, (void*)ctx->cpl_flux, (void*)ctx->cpl_snap, (void*)ctx->cpl_zsnap,
+                  (void*)ctx->cpl_dfirst})
     if (p) (void)hipFree(p);
   if (ctx->nsel_h) (void)hipHostFree(ctx->nsel_h);
   if (ctx->gs) (void)hipStreamSynchronize(ctx->gs);
@@ -961,6 +1233,11 @@ int uc_destroy(uc_context* ctx) {
 
 int uc_step(uc_context* ctx, const uc_field* f, const uc_controls* ctl, void* stream, uc_stats* st) {
   return step_impl(ctx, f, ctl, (hipStream_t)stream, st);
+}
+
+int uc_step_coupled(uc_context* ctx, const uc_field* f, const uc_controls* ctl, const uc_coupling* cp, void* stream,
+                    uc_stats* st) {
+  return step_cpl_impl(ctx, f, ctl, cp, (hipStream_t)stream, st);
 }
 
 int uc_step_batch(uc_context* ctx, const uc_field* fields, const uc_controls* ctls, int32_t ngrids, void* stream,

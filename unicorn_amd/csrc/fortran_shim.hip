@@ -18,11 +18,14 @@
 //   solveh_   enthalpy relaxation to the global stop and T(h) (solveh.f:1147-1234).
 //             Writes /DUMY/ FPZ, /CB04/ TK, /HRLS/ QDOT, /DUMD/ QDSP and ISOR2.
 //
-// Contract: chemistry only.  The transport coefficients SPGSOLV assembles
-// (/DUMMY/ and /DUM0/ E,W,P,Q of every species, /DUMX/ ZE..ZS, /DUMS/ STE..STS) must
-// be zero at every cell the chemistry advances; otherwise the program stops with a
-// message (the neighbour-coupled relaxation is not implemented; there is no CPU
-// fallback).  Errors of the device layer also stop the program, after printing
+// Coupling.  SPGSOLV's coefficient assembly stays in the Fortran; CKRATES reads what it
+// left in the COMMON blocks: /DUMMY/ and /DUM0/ E,W,P,Q of every species, /DUMS/
+// STE..STS, /DUMX/ ZE..ZS, the RDHSP inputs (/DUMY/ EMU, EKT, /CB07/ VDF, TMU), the
+// geometry (/CB03/, ISYM) and the velocities (/CB04/ U, V, W).  Where they are zero at
+// every cell the chemistry advances (the flux inputs: everywhere), the step takes the
+// chemistry-only path; otherwise the coupled one (uc_step_coupled's phases: the
+// reference's Gauss-Seidel order as a wavefront), exactly as the reference.  There is no
+// CPU fallback: errors of the device layer stop the program after printing
 // uc_last_error().  Cells with 3 <= ISKIP < 10 keep their QDOT/QDSP: the reference
 // evaluates them there from RC values CKRATES did not compute in that step.
 #include "chem_kernels.hip"
@@ -46,7 +49,7 @@ constexpr int LSP = 52;
 extern "C" {
 extern struct { int32_t isym, iflow, iswrl, ithrm, ichem, igrav, itr; int32_t iskip[shim::LE]; } cb01_;
 extern struct { double rref, vref, tref, pref, wm[shim::LSP]; } cb02_;
-extern struct { double dt; } cb03_;
+extern struct { double dt, x[shim::LE], y[shim::LE], xxc[shim::LI], xxs[shim::LI], yyc[shim::LJ], yys[shim::LJ]; } cb03_;
 extern struct { double rho[shim::LE], fsp[shim::LSP][shim::LE], u[shim::LE], v[shim::LE], w[shim::LE],
                 p[shim::LE], ht[shim::LE], tk[shim::LE]; } cb04_;
 extern struct { double alstr, vstr, rstr, pstr, tstr, tmstr, amstr, acstr, dstr, hstr, cpstr, akstr,
@@ -84,6 +87,12 @@ struct Shim {
   int32_t* iskip = nullptr;
   uc_field f{};
   int stage = 0;  // 1 after ckrates_, 2 after solvesp_
+  // coupling inputs on the device (allocated at first use) and what this step uses
+  double *coef = nullptr, *stc = nullptr, *zc = nullptr, *emu = nullptr, *ekt = nullptr, *tmu = nullptr,
+         *vdf = nullptr, *ygeo = nullptr, *xxc = nullptr, *xxs = nullptr, *yyc = nullptr, *yys = nullptr,
+         *u = nullptr, *v = nullptr, *w = nullptr;
+  uc_coupling cp{};
+  bool coupled = false;
 };
 Shim g;
 
@@ -138,35 +147,32 @@ bool active(int64_t c) {
   return i > 0 && i < shim::LI - 1 && j > 0 && j < shim::LJ - 1 && cb01_.iskip[c] < 3;
 }
 
-// the chemistry-only contract: no transport coefficient at any cell the chemistry advances
-void check_contract() {
+// Which coupling inputs this step has (host COMMONs only): coefficients and velocities
+// matter where the chemistry advances; the flux inputs (RDHSP = (EMU VDF + REI TMU/SIGSP -
+// EKT) HSSP, solveh.f:52-53) everywhere, because the flux reads neighbours.
+struct Present {
+  bool coef = false, st = false, z = false, flux = false, vel = false;
+};
+
+Present scan_coupling() {
+  Present p;
   auto nonzero = [](const double* a) {
     for (int64_t c = 0; c < shim::LE; ++c)
       if (a[c] != 0.0 && active(c)) return true;
     return false;
   };
-  for (int k = 0; k < 55; ++k)
-    if (nonzero(dummy_.a[k])) stop("ckrates_", "non-zero transport coefficient in /DUMMY/ (array " +
-                                                std::to_string(k + 1) + "): only the chemistry-only contract is implemented");
-  for (int k = 0; k < 149; ++k)
-    if (nonzero(dum0_.a[k])) stop("ckrates_", "non-zero transport coefficient in /DUM0/ (array " +
-                                               std::to_string(k + 1) + "): only the chemistry-only contract is implemented");
+  for (int k = 0; k < 55 && !p.coef; ++k) p.coef = nonzero(dummy_.a[k]);
+  for (int k = 0; k < 149 && !p.coef; ++k) p.coef = nonzero(dum0_.a[k]);
   for (int k = 0; k < 4; ++k) {
-    if (nonzero(dumx_.z[k])) stop("ckrates_", "non-zero enthalpy coefficient in /DUMX/");
-    if (nonzero(dums_.st[k])) stop("ckrates_", "non-zero soot coefficient in /DUMS/");
+    p.st = p.st || nonzero(dums_.st[k]);
+    p.z = p.z || nonzero(dumx_.z[k]);
   }
-  // RDHSP = (EMU VDF + REI TMU / SIGSP - EKT) HSSP (solveh.f:52-53) feeds the species-
-  // diffusion enthalpy flux of RHSZ (solveh.f:59-115) through every cell's neighbours;
-  // this drop-in computes RHSZ with that flux zero, so RDHSP must vanish everywhere:
-  // EKT = TMU = 0, and VDF = 0 wherever EMU is not (SPGSOLV sets EMU even without
-  // transport)
-  for (int64_t c = 0; c < shim::LE; ++c) {
-    bool flux = dumy_.ekt[c] != 0.0 || cb07_.tmu[c] != 0.0;
-    for (int k = 0; k < shim::LSP && !flux && dumy_.emu[c] != 0.0; ++k) flux = cb07_.vdf[k][c] != 0.0;
-    if (flux)
-      stop("ckrates_", "non-zero species-diffusion enthalpy flux input (EKT, TMU, or VDF with EMU): the flux "
-                       "term of SOLVEH is not implemented, only the chemistry-only contract");
+  p.vel = nonzero(cb04_.u) || nonzero(cb04_.v) || nonzero(cb04_.w);
+  for (int64_t c = 0; c < shim::LE && !p.flux; ++c) {
+    p.flux = dumy_.ekt[c] != 0.0 || cb07_.tmu[c] != 0.0;
+    for (int k = 0; k < shim::LSP && !p.flux && dumy_.emu[c] != 0.0; ++k) p.flux = cb07_.vdf[k][c] != 0.0;
   }
+  return p;
 }
 
 void h2d(void* dst, const void* src, size_t b, const char* where) {
@@ -182,7 +188,7 @@ extern "C" {
 
 // SUBROUTINE CKRATES (src/lib/ckrates.f:1)
 void ckrates_(void) {
-  check_contract();  // host COMMONs only: refuse before touching the GPU
+  const Present pr = scan_coupling();  // host COMMONs only
   ensure_context();
   const size_t b1 = sizeof(double) * shim::LE, b52 = b1 * shim::LSP;
   h2d(g.rho, cb04_.rho, b1, "ckrates_");
@@ -198,11 +204,47 @@ void ckrates_(void) {
   h2d(g.sst + shim::LE, dums_.sst2, b1, "ckrates_");
   h2d(g.qdot, hrls_.qdot, b1, "ckrates_");
   h2d(g.qdsp, dumd_.qdsp, b52, "ckrates_");
+  // the coupling inputs this step has
+  auto dev = [&](double*& p, size_t bytes) {
+    if (!p) hipcheck(hipMalloc(&p, bytes), "ckrates_: hipMalloc");
+    return p;
+  };
+  g.cp = uc_coupling{};
+  if (pr.coef) {
+    dev(g.coef, 204 * b1);
+    h2d(g.coef, dummy_.a, 55 * b1, "ckrates_");   // EA0..PAD
+    h2d(g.coef + 55 * shim::LE, dum0_.a, 149 * b1, "ckrates_");  // QAD..QCA
+    g.cp.coef = g.coef;
+  }
+  if (pr.st) { h2d(dev(g.stc, 4 * b1), dums_.st, 4 * b1, "ckrates_"); g.cp.stc = g.stc; }
+  if (pr.z) { h2d(dev(g.zc, 4 * b1), dumx_.z, 4 * b1, "ckrates_"); g.cp.zc = g.zc; }
+  if (pr.flux) {
+    h2d(dev(g.emu, b1), dumy_.emu, b1, "ckrates_");
+    h2d(dev(g.ekt, b1), dumy_.ekt, b1, "ckrates_");
+    h2d(dev(g.tmu, b1), cb07_.tmu, b1, "ckrates_");
+    h2d(dev(g.vdf, b52), cb07_.vdf, b52, "ckrates_");
+    h2d(dev(g.ygeo, b1), cb03_.y, b1, "ckrates_");
+    h2d(dev(g.xxc, sizeof(double) * shim::LI), cb03_.xxc, sizeof(double) * shim::LI, "ckrates_");
+    h2d(dev(g.xxs, sizeof(double) * shim::LI), cb03_.xxs, sizeof(double) * shim::LI, "ckrates_");
+    h2d(dev(g.yyc, sizeof(double) * shim::LJ), cb03_.yyc, sizeof(double) * shim::LJ, "ckrates_");
+    h2d(dev(g.yys, sizeof(double) * shim::LJ), cb03_.yys, sizeof(double) * shim::LJ, "ckrates_");
+    g.cp.emu = g.emu; g.cp.ekt = g.ekt; g.cp.tmu = g.tmu; g.cp.vdf = g.vdf;
+    g.cp.ygeo = g.ygeo; g.cp.xxc = g.xxc; g.cp.xxs = g.xxs; g.cp.yyc = g.yyc; g.cp.yys = g.yys;
+  }
+  if (pr.vel) {
+    h2d(dev(g.u, b1), cb04_.u, b1, "ckrates_");
+    h2d(dev(g.v, b1), cb04_.v, b1, "ckrates_");
+    h2d(dev(g.w, b1), cb04_.w, b1, "ckrates_");
+    g.cp.u = g.u; g.cp.v = g.v; g.cp.w = g.w;
+  }
+  g.cp.sigsp = 0.9;  // SOLVEH's argument, set there
+  g.cp.isym = cb01_.isym;
+  g.coupled = pr.coef || pr.st || pr.z || pr.flux || pr.vel;
   hipcheck(hipStreamSynchronize(g.s), "ckrates_");
   g.ctx->lay.ngrids = 0;  // ISKIP may change between steps: rebuild the cell list
   // SOLVESP/SOLVEH supply the controls and norms; these are placeholders until then
   const uc_controls ctl{cb03_.dt, 0.99, 1e-8, 0.99, 1e-8, 100, 100, UC_MODE_GLOBAL, 0};
-  check(ph_begin(g.ctx, &g.f, &ctl, 1, g.s, false, nullptr, nullptr), "ckrates_");
+  check(ph_begin(g.ctx, &g.f, &ctl, 1, g.s, false, nullptr, nullptr, g.coupled ? &g.cp : nullptr), "ckrates_");
   check(ph_rates(g.ctx), "ckrates_");
   g.stage = 1;
 }
@@ -222,6 +264,19 @@ void solvesp_(int32_t* isor1, double* relxsp, double* tolrsp, double* spnorm) {
   G.sor2 = 1.0 - *relxsp;
   G.tolrsp = *tolrsp;
   G.isor1 = *isor1;
+  const size_t b1 = sizeof(double) * shim::LE;
+  if (R.cpl_sp) {  // the closed iterates of every cell are in the field-layout iterate array
+    check(ph_species_cpl(g.ctx), "solvesp_");
+    *isor1 = R.out[0].isor1;
+    const double* it = g.ctx->cpl_it;
+    d2h(dumy_.ssp, it, b1 * shim::LSP, "solvesp_");
+    d2h(cb04_.fsp[51], it + 51 * shim::LE, b1, "solvesp_");  // FCB = SCB (solvesp.f:2912)
+    d2h(dums_.sst1, it + 52 * shim::LE, b1, "solvesp_");
+    d2h(dums_.sst2, it + 53 * shim::LE, b1, "solvesp_");
+    hipcheck(hipStreamSynchronize(g.s), "solvesp_");
+    g.stage = 2;
+    return;
+  }
   check(ph_species(g.ctx), "solvesp_");
   *isor1 = R.out[0].isor1;
   const int64_t npad = g.ctx->lay.npad, nb = blocks_for(npad, 256);
@@ -231,7 +286,6 @@ void solvesp_(int32_t* isor1, double* relxsp, double* tolrsp, double* spnorm) {
   k_scatter_rows<<<nb, 256, 0, g.s>>>(g.ctx->sa + 51 * npad, npad, g.ctx->cells, 1, g.y + 51 * shim::LE,
                                       shim::LE);
   hipcheck(hipGetLastError(), "solvesp_");
-  const size_t b1 = sizeof(double) * shim::LE;
   d2h(dumy_.ssp, g.ssp, b1 * shim::LSP, "solvesp_");
   d2h(cb04_.fsp[51], g.y + 51 * shim::LE, b1, "solvesp_");
   d2h(dums_.sst1, g.sst, b1, "solvesp_");
@@ -241,9 +295,8 @@ void solvesp_(int32_t* isor1, double* relxsp, double* tolrsp, double* spnorm) {
 }
 
 // SUBROUTINE SOLVEH(ISOR2,RELXH,TOLRH,SIGSP,HNORM) (src/lib/solveh.f:1); SIGSP enters
-// only the species-diffusion enthalpy flux, zero under the chemistry-only contract
+// the species-diffusion enthalpy flux (solveh.f:52-53)
 void solveh_(int32_t* isor2, double* relxh, double* tolrh, double* sigsp, double* hnorm) {
-  (void)sigsp;
   if (g.stage != 2) stop("solveh_", "called without a preceding SOLVESP");
   StepRun& R = g.ctx->run;
   if (*isor2 < 1 || *isor2 > 1000) stop("solveh_", "ISOR2 must be 1..1000");
@@ -257,7 +310,12 @@ void solveh_(int32_t* isor2, double* relxh, double* tolrh, double* sigsp, double
   G.zsor2 = 1.0 - *relxh;
   G.tolrh = *tolrh;
   G.isor2 = *isor2;
-  check(ph_enthalpy(g.ctx), "solveh_");
+  if (R.cpl_sp) {  // the sources with the enthalpy flux, which needs SIGSP
+    if (R.cpl_flux && !(*sigsp > 0.0)) stop("solveh_", "SIGSP must be positive");
+    R.cp.sigsp = *sigsp;
+    check(ph_sources_cpl(g.ctx), "solveh_");
+  }
+  check(R.cpl_h ? ph_enthalpy_cpl(g.ctx) : ph_enthalpy(g.ctx), "solveh_");
   check(ph_finish(g.ctx, g.fpz), "solveh_");
   uc_stats st{};
   check(ph_end(g.ctx, &st), "solveh_");

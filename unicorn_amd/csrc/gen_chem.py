@@ -208,7 +208,13 @@ def reactants(t):
     return reac
 
 
-SLOT_BASE = {"UC_SLOT_RHOFA": 544, "UC_SLOT_G": 595, "UC_SLOT_SOUR": 647, "UC_SLOT_SO": 653}
+SLOT_BASE = {"UC_SLOT_RHOFA": 544, "UC_SLOT_G": 595, "UC_SLOT_SOUR": 647, "UC_SLOT_SO": 653,
+             "UC_CPL": 1000, "UC_NBR": 2000}
+# the coupled sweep's reads beyond the slab (RcCpl in chem_kernels.hip):
+#   UC_CPL + 5*g + {0: PPP, 1: P, 2: Q, 3: E, 4: W} of group g (species 1..51 = g 0..50,
+#   soot scalars g 51, enthalpy g 52), from the cell's coupling tile;
+#   UC_NBR + 4*row + {0: J+1, 1: J-1, 2: I+1, 3: I-1}: neighbour iterate of row `row`
+CPL_SLOTS = 5 * 53 + 1
 RC_READ = re.compile(r"rc\.ld\(([^()]*)\)")
 
 # register cache and prefetch depth of the sweep's slab reads (doubles)
@@ -408,7 +414,17 @@ def emit_cached(stmts, L, K, M, KL=0, indent="  ", barrier="UC_SB();", transform
     return len(reads), nlds, len(loads)
 
 
-def emit_sweep(t, L):
+def neighbour_num(rhs, g, row):
+    """RHS1-P*S(J+1)-Q*S(J-1)-E*S(I+1)-W*S(I-1) of coefficient group g and iterate row
+    `row`, left to right (solvesp.f:425-426)."""
+    cb, nb = 5 * g, 4 * row
+    e = rhs
+    for i in range(4):
+        e = f"({e} - rc.ld(UC_CPL + {cb + 1 + i}) * rc.ld(UC_NBR + {nb + i}))"
+    return e[1:-1]
+
+
+def emit_sweep(t, L, coupled=False):
     reac = reactants(t)
     blocks = [b for b in t["sweep"]["blocks"] if b["kind"] != "soot"]
     stale_expected = {(x["block"], x["slot"]) for x in t["sweep"]["stale_doubled"]}
@@ -418,6 +434,12 @@ def emit_sweep(t, L):
         for j in set(rs):
             if rs.count(j) == 2:
                 doubled_species.add(j)
+    name = "uc_sweep_cpl" if coupled else "uc_sweep"
+    if coupled:
+        L.append("// uc_sweep with the neighbour terms of every relaxed scalar (solvesp.f:424-426):")
+        L.append("// FNEW = SOR2*S + (RHS1-P*S(J+1)-Q*S(J-1)-E*S(I+1)-W*S(I-1))*SOR1/(PPP-RHS2), PPP =")
+        L.append("// RHO-(E+W+P+Q) precomputed; coefficients and neighbour iterates read through rc")
+        L.append("// (UC_CPL / UC_NBR slots).  Same statement order, slab plan and caching as uc_sweep.")
     L.append("// one Gauss-Seidel sweep of one cell (solvesp.f:87-2831), chemistry-only contract.")
     L.append("// s[0..51]: species iterates, s[52] soot mass fraction, s[53] soot number density.")
     L.append("// Returns sum |S - FNEW| over species 1..51 and the soot mass fraction (RSD's")
@@ -427,11 +449,12 @@ def emit_sweep(t, L):
     L.append(f"// else in the lane's {SWEEP_LDS} LDS positions (farthest-next-use replacement at both")
     L.append(f"// levels, planned here); global loads run {SWEEP_PREFETCH} ahead of use.  lds: this lane's")
     L.append("// LDS positions, stride 64 doubles.")
-    L.append(f"#define UC_SWEEP_LDS {SWEEP_LDS}")
-    L.append("#define UC_LDS_LD(pos) (lds[(pos) * 64])")
-    L.append("#define UC_LDS_ST(pos, v) (lds[(pos) * 64] = (v))")
+    if not coupled:
+        L.append(f"#define UC_SWEEP_LDS {SWEEP_LDS}")
+        L.append("#define UC_LDS_LD(pos) (lds[(pos) * 64])")
+        L.append("#define UC_LDS_ST(pos, v) (lds[(pos) * 64] = (v))")
     L.append("template <class RcPtr>")
-    L.append("__device__ __forceinline__ double uc_sweep(UcP P, RcPtr rc, double* __restrict__ lds, double (&s)[54],")
+    L.append(f"__device__ __forceinline__ double {name}(UcP P, RcPtr rc, double* __restrict__ lds, double (&s)[54],")
     L.append("    double rho, double stmf, double stnd, double dt, double sor1, double sor2, double* dsst2) {")
     L.append("  double rsd = 0.0, rhs1a = 0.0;")
     so_slot = {j: i for i, j in enumerate(sorted(doubled_species))}
@@ -477,10 +500,16 @@ def emit_sweep(t, L):
                 c = f"{f32(term['coef'])} * " if term.get("coef") else ""
                 lt.append(f"{c}rc.ld(UC_SLOT_SOUR + {term['sour'] - 1})")
         chunked(f"loss_{k}", lt)
+        if coupled:
+            upd = (f" const double num_{k} = {neighbour_num(f'rhs1_{k}', k - 1, k - 1)};"
+                   f" const double fnew_{k} = sor2 * s[{k - 1}] + num_{k} * sor1"
+                   f" / (rc.ld(UC_CPL + {5 * (k - 1)}) - rhs2_{k});")
+        else:
+            upd = f" const double fnew_{k} = sor2 * s[{k - 1}] + rhs1_{k} * sor1 / (rho - rhs2_{k});"
         S.append(f"const double dtw_{k} = dt * P->wm[{k - 1}];"
                  f" const double rhs1_{k} = rc.ld(UC_SLOT_RHOFA + {k - 1}) + dtw_{k} * (rhs1a + rhs1b_{k});"
                  f" const double rhs2_{k} = -dtw_{k} * loss_{k};"
-                 f" const double fnew_{k} = sor2 * s[{k - 1}] + rhs1_{k} * sor1 / (rho - rhs2_{k});"
+                 + upd +
                  f" rsd += fabs(s[{k - 1}] - fnew_{k}); s[{k - 1}] = fnew_{k} < 0.0 ? 0.0 : fnew_{k};"
                  " asm volatile(\"\" : \"+v\"(rsd));")
         done.add(k)
@@ -489,12 +518,18 @@ def emit_sweep(t, L):
     assert not missing and not extra, (missing, extra)
     # soot (solvesp.f:2807-2831)
     sr = [f"rc.ld(UC_SLOT_SOUR + {q})" for q in range(6)]
+    # soot: PPP - RHS2 with RHS2 = 0 is PPP; with RHS2 = -DT*SOUR6 it is PPP + DT*SOUR6
+    if coupled:
+        num_sm, ppp_sm = neighbour_num("rhs1_soot", 51, 52), "rc.ld(UC_CPL + 255)"
+        num_sn, ppp_sn = neighbour_num("rhs1n", 51, 53), "rc.ld(UC_CPL + 255)"
+    else:
+        num_sm, ppp_sm, num_sn, ppp_sn = "rhs1_soot", "rho", "rhs1n", "rho"
     S.append(f"const double rhs1_soot = rho * stmf + dt * ({f32('0.012011')} / P->wmstr) * (2.0 * {sr[0]} * s[21]"
              f" - {sr[2]} * s[1] - {sr[3]} * s[4] - {sr[4]} * s[3]);"
-             " const double fnew_sm = sor2 * s[52] + rhs1_soot * sor1 / rho;"
+             f" const double fnew_sm = sor2 * s[52] + ({num_sm}) * sor1 / {ppp_sm};"
              " rsd += fabs(s[52] - fnew_sm); s[52] = fnew_sm < 0.0 ? 0.0 : fnew_sm;")
     S.append(f"const double rhs1n = rho * stnd + dt * {sr[1]} * s[21];"
-             f" double fnew_sn = sor2 * s[53] + rhs1n * sor1 / (rho + dt * {sr[5]});"
+             f" double fnew_sn = sor2 * s[53] + ({num_sn}) * sor1 / ({ppp_sn} + dt * {sr[5]});"
              " fnew_sn = fnew_sn < 0.0 ? 0.0 : fnew_sn; *dsst2 = fabs(s[53] - fnew_sn); s[53] = fnew_sn;")
     nreads, nlds, nloads = emit_cached(S, L, SWEEP_CACHE, SWEEP_PREFETCH, SWEEP_LDS)
     L.append("  (void)lds;")
@@ -609,11 +644,15 @@ def main(out=None):
          "#define UC_SLOT_G 595",
          "#define UC_SLOT_SOUR 647",
          "#define UC_SLOT_SO 653",
-         "#define UC_SLAB_SLOTS 664", ""]
+         "#define UC_SLAB_SLOTS 664",
+         "#define UC_CPL 1000",
+         "#define UC_NBR 2000",
+         f"#define UC_CPL_SLOTS {CPL_SLOTS}", ""]
     emit_host_tables(t, L)
     emit_rates(t, L)
     emit_soot(L)
     emit_sweep(t, L)
+    emit_sweep(t, L, coupled=True)
     emit_prd(t, L)
     text = "\n".join(L) + "\n"
     dst = Path(out) if out else HERE / "chem_gen.inc"
