@@ -17,7 +17,10 @@ found -- one process per core.  With the trained surrogate weights present, the
 line also carries "surrogate": the gated surrogate with FP64 fallback over the same
 K steps from the same seeded state (cell-updates/s, speedup over the FP64 step,
 fallback rate) and its error against the FP64 run (one-step and chained max |dT|
-and max major-species error).
+and max major-species error).  And "coupled": the same field with transport
+coefficients (unicorn_amd.fields.synthetic_coupling, seed 3) through
+uc_step_coupled, the reference's Gauss-Seidel as a wavefront, with its serial CPU
+baseline (--workload coupled prints that measurement as the line).
 """
 from __future__ import annotations
 
@@ -42,6 +45,13 @@ HBM_PEAK = 8.0e12            # B/s
 # + 28 per Newton iteration + 18 per enthalpy sweep; 880 B in+out per cell.
 F_BASE, F_SWEEP, F_NEWTON, F_HSWEEP = 12760 + 650, 5629, 28, 18
 B_CELL = 880
+# a coupled cell-sweep adds 4 products and 4 subtractions per relaxed scalar (51 species,
+# SST1, SST2; PPP is formed once per step): solvesp.f:424-426
+F_SWEEP_CPL = F_SWEEP + 8 * 53
+# the coupled workload's coefficients: diffusion number 0.02 per face (dt*D/dx^2 for
+# dt = 1e-5 s, D = 2e-5 m^2/s, dx = 1e-4 m: laminar diffusion at this resolution), the
+# enthalpy flux 2e-4 of rho*HT, velocities up to 0.3 VREF
+CPL_RECIPE = {"strength": 0.02, "flux": 2e-4, "velocity": 0.3}
 
 
 def parse():
@@ -52,11 +62,13 @@ def parse():
     p.add_argument("--cpu-cells", type=int, default=16000, help="CPU baseline sample size (cells)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-surrogate", action="store_true", help="skip the gated-surrogate measurement")
+    p.add_argument("--no-coupled", action="store_true", help="skip the coupled-step measurement")
     p.add_argument("--li", type=int, default=1001)
     p.add_argument("--lj", type=int, default=401)
-    p.add_argument("--workload", choices=("field", "batch"), default="field",
+    p.add_argument("--workload", choices=("field", "batch", "coupled"), default="field",
                    help="field: the 1001x401 C7H16 field (the headline line); batch: 256 grids "
-                        "of uneven size, fuel and dt in one batched step (autotuned first)")
+                        "of uneven size, fuel and dt in one batched step (autotuned first); coupled: "
+                        "the field with transport coefficients (uc_step_coupled)")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB (5 times): the reference "
                         "kernel for scaling FETCH_SIZE / WRITE_SIZE counter readings")
@@ -158,10 +170,96 @@ def batch_bench(args):
     print(json.dumps(out))
 
 
+def coupled_cpu_baseline(nstar, nh):
+    """The reference Fortran's coupled step (flang build, oracle/_ref) on a 201 x 81 field
+    of the same recipe with the same coupling recipe, one step forced to the GPU's sweep
+    counts, one core (the coupled Gauss-Seidel is serial); scaled per cell-sweep."""
+    from oracle import refrun
+    from unicorn_amd.fields import flame_field, synthetic_coupling
+    if not refrun.available():
+        return None
+    f = flame_field(201, 81, "C7H16", 1, preignition_band=0.10)
+    cp = synthetic_coupling(f, seed=3, **CPL_RECIPE)
+    r = refrun.run(f, nsteps=1, isor1=int(nstar), isor2=int(nh), force=True, coupling=cp)
+    cells = (f.li - 2) * (f.lj - 2)
+    return {"t_s": r.t_chem, "cells": cells, "sweeps": int(nstar),
+            "s_per_cell_sweep": r.t_chem / (cells * nstar), "cores": 1}
+
+
+def coupled_bench(args, f0=None, cpu=True):
+    """The field with synthetic transport coefficients through uc_step_coupled."""
+    import torch
+
+    from unicorn_amd.chem import ChemStep, Controls, DeviceCoupling, DeviceField
+    from unicorn_amd.fields import flame_field, synthetic_coupling
+
+    f0 = f0 if f0 is not None else flame_field(args.li, args.lj, "C7H16", 1, preignition_band=0.10)
+    cp = synthetic_coupling(f0, seed=3, **CPL_RECIPE)
+    ncell = (f0.li - 2) * (f0.lj - 2)
+    dev, dcp = DeviceField(f0), DeviceCoupling(cp)
+    init = {k: getattr(dev, k).clone() for k in ("rho", "y", "tk", "ht", "stmf", "stnd")}
+    ctl = Controls(dt=f0.dt)
+    ctx = ChemStep(f0.n)
+    for _ in range(args.warmup):
+        ctx.step_coupled(dev, ctl, dcp)
+    for k, v in init.items():
+        getattr(dev, k).copy_(v)
+    ctx.close()
+    ctx = ChemStep(f0.n)
+    steps = max(1, min(args.steps, 10))   # at ~0.1 s a step, 10 chained steps are plenty
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = [ctx.step_coupled(dev, ctl, dcp) for _ in range(steps)]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms_sweep = sum(s.ms_sweeps for s in stats)
+    sweeps = sum(s.cell_sweeps for s in stats)
+    ndiag = (f0.li - 2) + (f0.lj - 2) - 1
+    wave_steps = sum(s.sweep_launches for s in stats)   # k_sweep_cpl launches = wavefront steps
+    achieved = F_SWEEP_CPL * sweeps / (ms_sweep * 1e-3)
+    out = {"metric": "cell-chem updates/s, coupled step (transport coefficients, exact Gauss-Seidel order)",
+           "value": ncell * steps / el, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
+           "warmup": args.warmup, "ms_per_step": el / steps * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (unicorn_amd.fields.config1 + synthetic_coupling seed 3, "
+                   "strength 0.02, flux 2e-4, velocity 0.3)",
+           "config": {"workload": f"{f0.li}x{f0.lj} C7H16 field, 10% pre-ignition band, dt=1e-5 s, E/W/P/Q of "
+                                  "every species, STE..STS, ZE..ZS, enthalpy flux inputs and velocities; "
+                                  f"{steps} chained steps, global stops", "cells": ncell},
+           "roofline": {"kernel": "k_sweep_cpl", "bound": "mfma", "achieved": achieved / 1e12,
+                        "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK, "traffic": None,
+                        "note": "latency bound, not throughput bound: the exact order puts ndiag + 2*(S-1) "
+                                "wavefront steps on the critical path, each one cell-sweep of one wave; "
+                                f"{F_SWEEP_CPL} FP64 ops per cell-sweep"},
+           "wavefront": {"species_ms_per_step": ms_sweep / steps,
+                         "us_per_wavefront_step": ms_sweep * 1e3 / max(1, wave_steps),
+                         "wavefront_steps_per_step": wave_steps / steps, "diagonals": ndiag},
+           "sweeps": {"global_species": [s.isor1 for s in stats], "global_enthalpy": [s.isor2 for s in stats],
+                      "first_segment": [s.first_segment for s in stats]}}
+    if cpu and not args.no_cpu:
+        nstar = int(np.median([min(s.isor1, 100) for s in stats]))
+        nh = int(np.median([min(s.isor2, 100) for s in stats]))
+        c = coupled_cpu_baseline(nstar, nh)
+        if c:
+            gpu_sweeps_per_cell_step = sweeps / (ncell * steps)
+            v = 1.0 / (c["s_per_cell_sweep"] * gpu_sweeps_per_cell_step)
+            out["cpu_baseline"] = {"value": v, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                                   "sample": f"reference Fortran (flang -O2), one coupled step of a 201x81 field of "
+                                             f"the same recipe, forced to {nstar}/{nh} sweeps ({c['t_s']:.2f} s, "
+                                             f"{c['s_per_cell_sweep'] * 1e6:.2f} us per cell-sweep), scaled to "
+                                             f"this run's sweeps per cell-step; serial, as the reference's "
+                                             f"Gauss-Seidel is"}
+            out["gpu_over_cpu"] = out["value"] / v
+    return out
+
+
 def main():
     args = parse()
     if args.workload == "batch":
         batch_bench(args)
+        return
+    if args.workload == "coupled":
+        print(json.dumps(coupled_bench(args)))
         return
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -242,6 +340,12 @@ def main():
             out["surrogate"]["dtype"] = "bf16 network (fp32 accumulation), fp64 gate and fallback"
         else:
             out["surrogate"] = None
+    if not args.no_coupled:
+        c = coupled_bench(args, f0)
+        out["coupled"] = {k: c[k] for k in ("value", "ms_per_step", "roofline", "wavefront", "sweeps")}
+        if "cpu_baseline" in c:
+            out["coupled"]["cpu_baseline"] = c["cpu_baseline"]
+            out["coupled"]["gpu_over_cpu"] = c["gpu_over_cpu"]
     if not args.no_cpu:
         cpu = cpu_baseline(f0, nstars, nhs, args.cpu_cells, args.steps)
         if cpu:

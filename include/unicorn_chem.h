@@ -86,7 +86,8 @@ typedef struct {
                                snapshot window of the last segment */
     double ms_rates, ms_sweeps, ms_sources, ms_total; /* device time: rate coefficients,
                                sweep segments, sources; whole step */
-    int32_t sweep_launches; /* sweep-segment launches this step */
+    int32_t sweep_launches; /* sweep-segment launches this step (coupled step: wavefront steps,
+                               one k_sweep_cpl launch each) */
     int32_t first_segment;  /* sweeps in the first segment (the previous count minus one) */
 } uc_stats;
 

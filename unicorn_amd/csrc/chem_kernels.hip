@@ -1,8 +1,10 @@
 // MI355X (gfx950) kernels and host C ABI of UNICORN's chemistry step.
 //
 // Step = main:821-828 clip/closure -> CKRATES -> SOLVESP -> SOLVEH -> SPGSOLV
-// epilogue -> main:1056-1073 density, for the chemistry-only contract, on one
-// grid or a batch of grids (each with its own DT, controls and global stops).
+// epilogue -> main:1056-1073 density, on one grid or a batch of grids (each with its
+// own DT, controls and global stops); or, with the transport coefficients SPGSOLV
+// assembles, on one grid in the reference's Gauss-Seidel order (k_sweep_cpl and
+// friends, below k_zero).
 //
 // Kernels (FP64 throughout; one thread per position of the cell list, whose
 // grids are padded to whole waves):

@@ -221,6 +221,11 @@ RC_READ = re.compile(r"rc\.ld\(([^()]*)\)")
 SWEEP_CACHE = int(os.environ.get("UC_SWEEP_CACHE", "70"))
 SWEEP_PREFETCH = int(os.environ.get("UC_SWEEP_PREFETCH", "32"))
 SWEEP_LDS = int(os.environ.get("UC_SWEEP_LDS", "126"))     # LDS positions (doubles per lane)
+# the coupled sweep runs a few waves per wavefront step, so it is bound by load latency,
+# not bandwidth: fewer values cached in registers, more loads in flight
+CPL_CACHE = int(os.environ.get("UC_CPL_CACHE", str(SWEEP_CACHE)))
+CPL_PREFETCH = int(os.environ.get("UC_CPL_PREFETCH", str(SWEEP_PREFETCH)))
+CPL_BARRIER = int(os.environ.get("UC_CPL_BARRIER", "1"))   # statements between scheduling barriers
 
 
 def slot_value(expr: str) -> int:
@@ -356,7 +361,7 @@ def plan_two_level(reads, K, KL, stmt_of=None):
     return best
 
 
-def emit_cached(stmts, L, K, M, KL=0, indent="  ", barrier="UC_SB();", transform=None):
+def emit_cached(stmts, L, K, M, KL=0, indent="  ", barrier="UC_SB();", transform=None, barrier_every=1):
     """Emit straight-line statements whose slab reads rc.ld(slot) are served from
     registers and the wave's LDS as plan_two_level decides.  Global loads are issued
     up to M loads ahead of their first use; a value kept in LDS is stored after the
@@ -410,7 +415,8 @@ def emit_cached(stmts, L, K, M, KL=0, indent="  ", barrier="UC_SB();", transform
         for j in by_stmt[i]:
             for pos in stores.get(j, ()):
                 L.append(f"{indent}UC_LDS_ST({pos}, {var[j]});")
-        L.append(indent + barrier)
+        if (i + 1) % barrier_every == 0 or i + 1 == len(tmpl):
+            L.append(indent + barrier)
     return len(reads), nlds, len(loads)
 
 
@@ -445,9 +451,10 @@ def emit_sweep(t, L, coupled=False):
     L.append("// Returns sum |S - FNEW| over species 1..51 and the soot mass fraction (RSD's")
     L.append("// per-cell share); *dsst2 receives |dSST2| (not in RSD).  Slab reads: RC, rho*Y_k(old)")
     L.append("// (UC_SLOT_RHOFA+k), SOUR (UC_SLOT_SOUR+q); DT*W_k formed as the reference does.  Each slab value is")
-    L.append(f"// loaded once and kept in a register while it is among the {SWEEP_CACHE} soonest reused,")
+    K, M = (CPL_CACHE, CPL_PREFETCH) if coupled else (SWEEP_CACHE, SWEEP_PREFETCH)
+    L.append(f"// loaded once and kept in a register while it is among the {K} soonest reused,")
     L.append(f"// else in the lane's {SWEEP_LDS} LDS positions (farthest-next-use replacement at both")
-    L.append(f"// levels, planned here); global loads run {SWEEP_PREFETCH} ahead of use.  lds: this lane's")
+    L.append(f"// levels, planned here); global loads run {M} ahead of use.  lds: this lane's")
     L.append("// LDS positions, stride 64 doubles.")
     if not coupled:
         L.append(f"#define UC_SWEEP_LDS {SWEEP_LDS}")
@@ -531,7 +538,7 @@ def emit_sweep(t, L, coupled=False):
     S.append(f"const double rhs1n = rho * stnd + dt * {sr[1]} * s[21];"
              f" double fnew_sn = sor2 * s[53] + ({num_sn}) * sor1 / ({ppp_sn} + dt * {sr[5]});"
              " fnew_sn = fnew_sn < 0.0 ? 0.0 : fnew_sn; *dsst2 = fabs(s[53] - fnew_sn); s[53] = fnew_sn;")
-    nreads, nlds, nloads = emit_cached(S, L, SWEEP_CACHE, SWEEP_PREFETCH, SWEEP_LDS)
+    nreads, nlds, nloads = emit_cached(S, L, K, M, SWEEP_LDS, barrier_every=CPL_BARRIER if coupled else 1)
     L.append("  (void)lds;")
     L.append("  return rsd;")
     L.append("}")
