@@ -343,23 +343,24 @@ def generate_pairs(npairs: int, seed: int = 7, steps_per_field: int = 200, round
     return Dataset(xraw, traw, rows, float(sm[0]), float(sm[1]))
 
 
-def target_scales(ds: Dataset, sample: int = 1 << 18, seed: int = 0) -> np.ndarray:
-    """Per-target compression scale: the median |change| over a seeded sample of rows
-    (floored at 1e-6 of the largest |change| in the sample, and at 1e-30)."""
+def target_scales(ds: Dataset, sample: int = 1 << 18, seed: int = 0, quantile: float = 0.5) -> np.ndarray:
+    """Per-target compression scale: the given quantile of |change| over a seeded sample
+    of rows (the median by default; floored at 1e-6 of the largest |change| in the
+    sample, and at 1e-30).  A lower quantile resolves small changes logarithmically."""
     import torch
     gen = torch.Generator(device="cuda").manual_seed(seed)
     idx = torch.randint(0, ds.rows, (min(sample, ds.rows),), device="cuda", generator=gen)
     a = ds.traw[idx].abs().double().cpu().numpy()
-    return np.maximum(np.maximum(np.median(a, axis=0), 1e-6 * a.max(axis=0)), 1e-30)
+    return np.maximum(np.maximum(np.quantile(a, quantile, axis=0), 1e-6 * a.max(axis=0)), 1e-30)
 
 
-def column_stats(ds: Dataset) -> dict:
+def column_stats(ds: Dataset, scale_quantile: float = 0.5) -> dict:
     """mean, sd, min, max of the transformed features and of the compressed targets
     asinh(d / scale), and the targets' scales."""
     import torch
     lib = library()
     out = {}
-    scale = target_scales(ds)
+    scale = target_scales(ds, quantile=scale_quantile)
     sc_dev = torch.from_numpy(scale.astype(np.float32)).cuda()
     for name, raw, ncol, kind in (("x", ds.xraw, NF, 0), ("t", ds.traw, NT, 1)):
         chunk = 1 << 16

@@ -38,11 +38,12 @@ def cmd_train(a):
     t0 = time.perf_counter()
     ds = S.generate_pairs(a.pairs, seed=a.seed, log=log)
     t_gen = time.perf_counter() - t0
-    stats = S.column_stats(ds)
+    stats = S.column_stats(ds, a.scale_quantile)
     net, info = S.train(ds, stats, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
     stats["stmf_max"] = ds.stmf_max
     stats["stnd_max"] = ds.stnd_max
-    S.save(a.out, net, stats, {"pairs": ds.rows, "epochs": a.epochs, "batch": a.batch, "lr": a.lr})
+    S.save(a.out, net, stats, {"pairs": ds.rows, "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
+                               "scale_quantile": a.scale_quantile})
     torch.cuda.synchronize()
     print(json.dumps({"train": {"pairs": ds.rows, "generate_s": t_gen, "train_s": info["seconds"],
                                 "final_train_mse": info["history"][-1][1], "final_val_mse": info["history"][-1][2],
@@ -150,6 +151,8 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--seed", type=int, default=7)
     t.add_argument("--out", type=Path, default=DEFAULT_WEIGHTS)
+    t.add_argument("--scale-quantile", type=float, default=0.5,
+                   help="quantile of |change| that sets each target's asinh compression scale")
     i = sub.add_parser("infer")
     i.add_argument("--weights", type=Path, default=DEFAULT_WEIGHTS)
     i.add_argument("--steps", type=int, default=200)
