@@ -63,6 +63,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-surrogate", action="store_true", help="skip the gated-surrogate measurement")
     p.add_argument("--no-coupled", action="store_true", help="skip the coupled-step measurement")
+    p.add_argument("--no-pcie", action="store_true", help="skip the PCIe-inclusive (uc_step_host) measurement")
     p.add_argument("--li", type=int, default=1001)
     p.add_argument("--lj", type=int, default=401)
     p.add_argument("--workload", choices=("field", "batch", "coupled"), default="field",
@@ -340,6 +341,22 @@ def main():
             out["surrogate"]["dtype"] = "bf16 network (fp32 accumulation), fp64 gate and fallback"
         else:
             out["surrogate"] = None
+    # PCIe-inclusive: uc_step_host copies the field in and out on every call (never `value`)
+    hsteps = 0 if args.no_pcie else 3
+    if hsteps:
+        hf = f0.copy()
+        hctx = ChemStep(f0.n)
+        t0 = time.perf_counter()
+        for _ in range(hsteps):
+            hctx.step_host(hf, ctl)
+        el_h = time.perf_counter() - t0
+        hctx.close()
+        out["pcie_inclusive"] = {"value": ncell * hsteps / el_h, "unit": "cell-updates/s",
+                                 "ms_per_step": el_h / hsteps * 1e3, "steps": hsteps,
+                                 "device_resident_ms_per_step_same_steps":
+                                     sum(st.ms_total for st in stats[:hsteps]) / max(1, min(hsteps, len(stats))),
+                                 "note": "uc_step_host from the seeded state: pageable host arrays copied in and "
+                                         "out every step (first steps, so more sweeps than the 20-step average)"}
     if not args.no_coupled:
         c = coupled_bench(args, f0)
         out["coupled"] = {k: c[k] for k in ("value", "ms_per_step", "roofline", "wavefront", "sweeps")}
