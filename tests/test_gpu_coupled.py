@@ -130,3 +130,27 @@ def test_gpu_velocities_enter_the_temperature(torch_cuda):
     plain = f0.copy()
     pyoracle.step(plain)
     assert np.abs(plain.tk[inter] - got.tk[inter]).max() > 100 * 1e-8 * np.abs(plain.tk[inter]).max()
+
+
+@pytest.mark.parametrize("controls", [dict(relxsp=0.6, relxh=0.3), dict(isor1=7, isor2=5),
+                                      dict(relxh=0.2, isor2=200)],
+                         ids=["two-segment-stops", "sweep-limits", "long-enthalpy"])
+def test_gpu_coupled_segments_and_limits(torch_cuda, controls):
+    """The coupled step's segment logic against the oracle over two chained steps: global
+    stops past one 48-sweep segment (87 then 42 species sweeps, 84 enthalpy sweeps), a
+    predicted first segment that falls short, the sweep limits reached (ISOR+1 returned),
+    and an enthalpy relaxation of 130 sweeps (three segments)."""
+    from unicorn_amd.chem import ChemStep, Controls, DeviceCoupling, DeviceField
+    from unicorn_amd.fields import flame_field, synthetic_coupling
+    f0 = flame_field(31, 23, "CH4", 12)
+    cp = synthetic_coupling(f0, seed=13)
+    ref = f0.copy()
+    outs = [pyoracle.step(ref, coupling=cp, **controls) for _ in range(2)]
+    print(f"coupled segments {controls}", flush=True)
+    dev, dcp, ctx = DeviceField(f0.copy()), DeviceCoupling(cp), ChemStep(f0.n)
+    stats = [ctx.step_coupled(dev, Controls(dt=f0.dt, **controls), dcp) for _ in range(2)]
+    assert [(s.isor1, s.isor2) for s in stats] == [(o.isor1, o.isor2) for o in outs]
+    got = dev.to_host()
+    inter = f0.interior()
+    e = {k: getattr(ref, k) for k in ("rho", "y", "tk", "ht", "stmf", "stnd")}
+    assert_state_close(got, e, inter, f"segments {controls}")
