@@ -9,7 +9,9 @@
 // given here at compile time (UC_LI, UC_LJ; src/lib/ckrates.f:4 has 711 x 131).
 //
 // Division of work (one call each per SPGSOLV call, in this order):
-//   ckrates_  upload the old-level state; rate coefficients, rho*Y and soot sources
+//   ckrates_  upload the old-level state (SA, SST1/SST2 and FPZ are taken as SPGSOLV's
+//             prologue leaves them, FA, STMF/STND and HT, spgsolv.f:169-231, and copied on
+//             the device); rate coefficients, rho*Y and soot sources
 //             into the device slabs (ckrates.f:92-1352).  RC stays on the device:
 //             only SOLVESP and SOLVEH read it (/DUMR/ is scratch elsewhere).
 //   solvesp_  sweeps to the global stop, N2 closure (solvesp.f:83-2914); also the
@@ -30,7 +32,9 @@
 // evaluates them there from RC values CKRATES did not compute in that step.
 #include "chem_kernels.hip"
 
+#include <atomic>
 #include <cstdlib>
+#include <thread>
 
 #ifndef UC_LI
 #define UC_LI 711
@@ -93,6 +97,11 @@ struct Shim {
          *u = nullptr, *v = nullptr, *w = nullptr;
   uc_coupling cp{};
   bool coupled = false;
+  // QDOT/QDSP on the device mirror the COMMONs once uploaded: only SOLVEH writes them
+  // (solveh.f:1138,1142; every other routine only reads them) and solveh_ downloads both
+  // whole, so later calls need not upload them again (UNICORN_FSHIM_UPLOAD_QDOT=1 in the
+  // environment uploads them on every call, for a program that writes them elsewhere)
+  bool q_resident = false;
 };
 Shim g;
 
@@ -128,6 +137,7 @@ void ensure_context() {
   if (g.ctx && std::memcmp(&sc, &g.sc, sizeof(sc)) == 0) return;
   if (g.ctx) check(uc_destroy(g.ctx), "ckrates_");
   g.sc = sc;
+  g.q_resident = false;
   check(uc_create(&g.sc, shim::LE, &g.ctx), "ckrates_");
   if (!g.s) {
     hipcheck(hipStreamCreate(&g.s), "ckrates_: hipStreamCreate");
@@ -154,24 +164,47 @@ struct Present {
   bool coef = false, st = false, z = false, flux = false, vel = false;
 };
 
+// A chemistry-only run reads every coefficient array to find them all zero (about 150 MB at
+// the reference grid), so the arrays are scanned by several threads.
 Present scan_coupling() {
-  Present p;
   auto nonzero = [](const double* a) {
     for (int64_t c = 0; c < shim::LE; ++c)
       if (a[c] != 0.0 && active(c)) return true;
     return false;
   };
-  for (int k = 0; k < 55 && !p.coef; ++k) p.coef = nonzero(dummy_.a[k]);
-  for (int k = 0; k < 149 && !p.coef; ++k) p.coef = nonzero(dum0_.a[k]);
-  for (int k = 0; k < 4; ++k) {
-    p.st = p.st || nonzero(dums_.st[k]);
-    p.z = p.z || nonzero(dumx_.z[k]);
-  }
-  p.vel = nonzero(cb04_.u) || nonzero(cb04_.v) || nonzero(cb04_.w);
-  for (int64_t c = 0; c < shim::LE && !p.flux; ++c) {
-    p.flux = dumy_.ekt[c] != 0.0 || cb07_.tmu[c] != 0.0;
-    for (int k = 0; k < shim::LSP && !p.flux && dumy_.emu[c] != 0.0; ++k) p.flux = cb07_.vdf[k][c] != 0.0;
-  }
+  // tasks 0..203: species coefficient arrays; 204..207 STE..STS; 208..211 ZE..ZS;
+  // 212..214 U, V, W; 215: the flux inputs
+  constexpr int NTASK = 216;
+  std::atomic<int> next{0};
+  std::atomic<bool> coef{false}, st{false}, z{false}, vel{false}, flux{false};
+  auto work = [&] {
+    for (int t; (t = next.fetch_add(1)) < NTASK;) {
+      if (t < 204) {
+        if (!coef && nonzero(t < 55 ? dummy_.a[t] : dum0_.a[t - 55])) coef = true;
+      } else if (t < 208) {
+        if (!st && nonzero(dums_.st[t - 204])) st = true;
+      } else if (t < 212) {
+        if (!z && nonzero(dumx_.z[t - 208])) z = true;
+      } else if (t < 215) {
+        const double* a = t == 212 ? cb04_.u : t == 213 ? cb04_.v : cb04_.w;
+        if (!vel && nonzero(a)) vel = true;
+      } else {
+        bool f = false;
+        for (int64_t c = 0; c < shim::LE && !f; ++c) {
+          f = dumy_.ekt[c] != 0.0 || cb07_.tmu[c] != 0.0;
+          for (int k = 0; k < shim::LSP && !f && dumy_.emu[c] != 0.0; ++k) f = cb07_.vdf[k][c] != 0.0;
+        }
+        if (f) flux = true;
+      }
+    }
+  };
+  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (unsigned i = 1; i < nt; ++i) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+  Present p;
+  p.coef = coef; p.st = st; p.z = z; p.vel = vel; p.flux = flux;
   return p;
 }
 
@@ -192,18 +225,27 @@ void ckrates_(void) {
   ensure_context();
   const size_t b1 = sizeof(double) * shim::LE, b52 = b1 * shim::LSP;
   h2d(g.rho, cb04_.rho, b1, "ckrates_");
-  h2d(g.y, cb04_.fsp, b52, "ckrates_");  // FA = SA on entry (spgsolv.f:169-220)
+  h2d(g.y, cb04_.fsp, b52, "ckrates_");
   h2d(g.tk, cb04_.tk, b1, "ckrates_");
   h2d(g.ht, dumy_.fpz, b1, "ckrates_");   // FPZ = HT on entry
-  h2d(g.fpz, dumy_.fpz, b1, "ckrates_");
   h2d(g.stmf, soot_.stmf, b1, "ckrates_");
   h2d(g.stnd, soot_.stnd, b1, "ckrates_");
   h2d(g.iskip, cb01_.iskip, sizeof(int32_t) * shim::LE, "ckrates_");
-  h2d(g.ssp, dumy_.ssp, b52, "ckrates_");
-  h2d(g.sst, dums_.sst1, b1, "ckrates_");
-  h2d(g.sst + shim::LE, dums_.sst2, b1, "ckrates_");
-  h2d(g.qdot, hrls_.qdot, b1, "ckrates_");
-  h2d(g.qdsp, dumd_.qdsp, b52, "ckrates_");
+  // SPGSOLV's prologue (spgsolv.f:169-231) set SA = FA, SST1 = STMF, SST2 = STND and
+  // FPZ = HT at every cell just before this call: those copies are made on the device
+  hipcheck(hipMemcpyAsync(g.ssp, g.y, b52, hipMemcpyDeviceToDevice, g.s), "ckrates_");
+  hipcheck(hipMemcpyAsync(g.sst, g.stmf, b1, hipMemcpyDeviceToDevice, g.s), "ckrates_");
+  hipcheck(hipMemcpyAsync(g.sst + shim::LE, g.stnd, b1, hipMemcpyDeviceToDevice, g.s), "ckrates_");
+  hipcheck(hipMemcpyAsync(g.fpz, g.ht, b1, hipMemcpyDeviceToDevice, g.s), "ckrates_");
+  static const bool q_always = [] {
+    const char* e = std::getenv("UNICORN_FSHIM_UPLOAD_QDOT");
+    return e && *e && *e != '0';
+  }();
+  if (!g.q_resident || q_always) {
+    h2d(g.qdot, hrls_.qdot, b1, "ckrates_");
+    h2d(g.qdsp, dumd_.qdsp, b52, "ckrates_");
+    g.q_resident = true;
+  }
   // the coupling inputs this step has
   auto dev = [&](double*& p, size_t bytes) {
     if (!p) hipcheck(hipMalloc(&p, bytes), "ckrates_: hipMalloc");
