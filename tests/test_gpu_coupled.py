@@ -154,3 +154,54 @@ def test_gpu_coupled_segments_and_limits(torch_cuda, controls):
     inter = f0.interior()
     e = {k: getattr(ref, k) for k in ("rho", "y", "tk", "ht", "stmf", "stnd")}
     assert_state_close(got, e, inter, f"segments {controls}")
+
+
+def test_gpu_coupled_five_chained_steps_match_oracle(torch_cuda):
+    """Five chained coupled steps (51 x 37 C2H4 field, soot, holes) stay within the parity
+    bar of the oracle, with the same sweep counts at every step."""
+    from unicorn_amd.fields import flame_field, synthetic_coupling
+    f0 = flame_field(51, 37, "C2H4", 41)
+    rng = np.random.default_rng(41)
+    sel = rng.random(f0.n) < 0.3
+    f0.stmf[sel] = 10.0 ** rng.uniform(-7, -4, sel.sum())
+    f0.stnd[sel] = 10.0 ** rng.uniform(12, 16, sel.sum())
+    f0.iskip[rng.choice(f0.interior(), size=30, replace=False)] = 10
+    cp = synthetic_coupling(f0, seed=42)
+    ref = f0.copy()
+    outs = [pyoracle.step(ref, coupling=cp) for _ in range(5)]
+    print("coupled five chained steps", [(o.isor1, o.isor2) for o in outs], flush=True)
+    dev, stats = coupled_run(f0.copy(), cp, 5)
+    assert [(s.isor1, s.isor2) for s in stats] == [(o.isor1, o.isor2) for o in outs]
+    got = dev.to_host()
+    inter = f0.interior()
+    e = {k: getattr(ref, k) for k in ("rho", "y", "tk", "ht", "stmf", "stnd")}
+    assert_state_close(got, e, inter, "five chained coupled steps")
+
+
+def test_gpu_coupled_full_size_invariants(torch_cuda):
+    """The 1001 x 401 C7H16 field with coupling (the bench's recipe), two steps: every
+    active cell closed (sum Y + STMF = 1 to rounding), Y >= 0, T within the Newton clamp,
+    the run repeated gives the same bits, and the coupling changes the result against the
+    chemistry-only step.  (The serial oracle takes minutes per step at this size.)"""
+    from unicorn_amd.chem import ChemStep, Controls, DeviceField
+    from unicorn_amd.fields import config1, synthetic_coupling
+    f0 = config1()
+    cp = synthetic_coupling(f0, seed=3, strength=0.02, flux=2e-4, velocity=0.3)
+    print("coupled full size", flush=True)
+    runs = [coupled_run(f0.copy(), cp, 2) for _ in range(2)]
+    (dev, stats), (dev2, stats2) = runs
+    assert [(s.isor1, s.isor2) for s in stats] == [(s.isor1, s.isor2) for s in stats2]
+    a, b = dev.to_host(), dev2.to_host()
+    assert np.array_equal(a.y, b.y) and np.array_equal(a.tk, b.tk) and np.array_equal(a.ht, b.ht)
+    cells = f0.active()
+    s = a.y[:, cells].sum(axis=0) + a.stmf[cells]
+    assert np.abs(s - 1.0).max() <= 1e-12
+    assert (a.y[:, cells] >= 0.0).all()
+    T = a.tk[cells] * 294.0
+    assert T.min() >= 293.0 - 1e-9 and T.max() <= 5000.0 + 1e-9
+    plain = DeviceField(f0.copy())
+    ctx = ChemStep(f0.n)
+    for _ in range(2):
+        ctx.step(plain, Controls(dt=f0.dt))
+    dT = np.abs(plain.to_host().tk[cells] - a.tk[cells]) * 294.0
+    assert dT.max() > 1.0, dT.max()
