@@ -17,17 +17,22 @@ namespace {
 
 constexpr int NSPS = 52;
 constexpr int NF = 54, NT = 53, NPAD = 64;
-// k_gate: threads per block (its LDS holds NT floats per thread, so smaller blocks let
-// more waves share a CU) and the unroll of its species loops (loads of later species
-// overlap the arithmetic of earlier ones).  Measured on the 1001 x 401 field: 64 / 4
-// takes the prediction from 1.37 to 1.25 ms.
+// k_gate: threads per block (a multiple of 64; four lanes per cell)
 #ifndef UC_GATE_BLOCK
-#define UC_GATE_BLOCK 64
+#define UC_GATE_BLOCK 256
 #endif
-#ifndef UC_GATE_UNROLL
-#define UC_GATE_UNROLL 4
+#ifndef UC_GATE_WAVES
+#define UC_GATE_WAVES 2
 #endif
+#define GATE_WAVES UC_GATE_WAVES  // waves per SIMD the gate's registers must allow (3 spills)
 constexpr int GATE_BLOCK = UC_GATE_BLOCK;
+// k_network: waves per block.  The block stages every weight k-step through LDS once
+// for all its waves (16 cells each), so more waves per block read the weights from L2
+// fewer times per cell.
+#ifndef UC_NET_WAVES
+#define UC_NET_WAVES 4
+#endif
+constexpr int NET_WAVES = UC_NET_WAVES;
 
 __device__ __forceinline__ double boxcox(double y) { return (pow(y > 0.0 ? y : 0.0, 0.1) - 1.0) / 0.1; }
 // inverse on the range of boxcox: (1 + 0.1 b)^10, 0 below b = -10
@@ -158,10 +163,8 @@ __global__ void k_features(const double* __restrict__ y, const double* __restric
 // the radical build-up before an ignition (a cell whose step changes fast goes to the
 // FP64 step, where the network's relative error would be a large absolute one); then
 // write Y, TK and HT (the sensible enthalpy of the new T and Y, whose inverse is
-// solveh.f:1170-1230).  Otherwise the cell is left as it was.  Two passes over the
-// species (the second recomputes the first's values with the same arithmetic) keep
-// the registers few.  The soot scalars are not predicted: an accepted cell keeps its
-// STMF and STND.  Returns 0 accepted, 1 out of range, 2 residual, 3 sum, 4 active;
+// solveh.f:1170-1230).  Otherwise the cell is left as it was.  The soot scalars are
+// not predicted: an accepted cell keeps its STMF and STND.  Returns 0 accepted, 1 out of range, 2 residual, 3 sum, 4 active;
 // rr: C/H/O residual of the raw prediction.
 // Output j of the network: u = v tsd + tmean (un-standardised, still compressed), in
 // float, the network's own precision; the change it stands for is d = tscale sinh(u).
@@ -176,8 +179,9 @@ __device__ __forceinline__ float out_change(float v, int j, const Norm* __restri
   return d;
 }
 
-// Y0^0.1, the base of the Box-Cox transform (Y < 0 taken as 0)
-__device__ __forceinline__ double bc_base(double y) { return pow(y > 0.0 ? y : 0.0, 0.1); }
+// Y0^0.1, the base of the Box-Cox transform (Y < 0 and NaN taken as 0), as
+// exp(0.1 ln Y): within a few ulp of pow, at a fraction of its cost (0 at Y = 0)
+__device__ __forceinline__ double bc_base(double y) { return exp(0.1 * log(y > 0.0 ? y : 0.0)); }
 
 // Y after a Box-Cox change d, from P = Y0^0.1: BC^-1(BC(Y0) + d) = (P + 0.1 d)^10 (0 when
 // the base falls below 0)
@@ -188,117 +192,81 @@ __device__ __forceinline__ double y_after(double P, float d) {
   return u8 * u2;
 }
 
-// The gate of one cell, one thread.  D d: d(j), the change of output j (j = 0: dT in K;
-// j = 1 + k: the Box-Cox change of Y_k), from out_change; PB pb: pb(k) = Y0_k^0.1.
-// flags_ok: the features lie in the hull and every output in range; slow_bc: every
-// species' Box-Cox change within tol_bc.  Species are summed in index order.
-template <class D, class PB>
-__device__ int gate_cell(D d, PB pb, const Thermo* __restrict__ th, double* __restrict__ y,
-                         double* __restrict__ tk, double* __restrict__ ht, double stm, double stn, int64_t n,
-                         int64_t c, bool flags_ok, bool slow_bc, double tol_el, double tol_sum, double tol_dt,
-                         double tol_dy, double* rr) {
-  // raw residual r = A dY and the weighted normal matrix M = A W A^T (symmetric 3x3)
-  double m[6] = {0, 0, 0, 0, 0, 0};
-  rr[0] = rr[1] = rr[2] = 0.0;
-#pragma unroll UC_GATE_UNROLL
-  for (int k = 0; k < 49; ++k) {
-    const double y0 = y[(int64_t)k * n + c], y1 = y_after(pb(k), d(1 + k));
-    const double w = fmax(y0, y1), a0 = th->mel[0][k], a1 = th->mel[1][k], a2 = th->mel[2][k];
-    rr[0] += a0 * (y1 - y0);
-    rr[1] += a1 * (y1 - y0);
-    rr[2] += a2 * (y1 - y0);
-    m[0] += w * a0 * a0; m[1] += w * a0 * a1; m[2] += w * a0 * a2;
-    m[3] += w * a1 * a1; m[4] += w * a1 * a2; m[5] += w * a2 * a2;
-  }
-  // lambda = M^-1 r (Cramer); the correction of species k is -w_k A_k^T lambda
-  const double c00 = m[3] * m[5] - m[4] * m[4], c01 = m[2] * m[4] - m[1] * m[5], c02 = m[1] * m[4] - m[2] * m[3];
-  const double c11 = m[0] * m[5] - m[2] * m[2], c12 = m[1] * m[2] - m[0] * m[4], c22 = m[0] * m[3] - m[1] * m[1];
-  const double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
-  double lam[3] = {0.0, 0.0, 0.0};
-  if (det > 0.0) {
-    lam[0] = (c00 * rr[0] + c01 * rr[1] + c02 * rr[2]) / det;
-    lam[1] = (c01 * rr[0] + c11 * rr[1] + c12 * rr[2]) / det;
-    lam[2] = (c02 * rr[0] + c12 * rr[1] + c22 * rr[2]) / det;
-  }
-  // Y1 of species k < 51: predicted, corrected, clipped (both passes use this)
-  auto y1_of = [&](int k, double y0) {
-    double y1 = y_after(pb(k), d(1 + k));
-    if (k < 49) y1 -= fmax(y0, y1) * (th->mel[0][k] * lam[0] + th->mel[1][k] * lam[1] + th->mel[2][k] * lam[2]);
-    return y1 < 0.0 ? 0.0 : y1;
-  };
-  double tot = 0.0, dm[3] = {0.0, 0.0, 0.0}, m1[3] = {0.0, 0.0, 0.0}, dmax = 0.0;
-#pragma unroll UC_GATE_UNROLL
-  for (int k = 0; k < 51; ++k) {
-    const double y0 = y[(int64_t)k * n + c], y1 = y1_of(k, y0);
-    tot += y1;
-    dmax = fmax(dmax, fabs(y1 - y0));
-    for (int e = 0; e < 3; ++e) {
-      dm[e] += th->mel[e][k] * (y1 - y0);
-      m1[e] += th->mel[e][k] * y1;
-    }
-  }
-  const double ftot = tot + stm;
-  const bool over = ftot > 1.0;
-  const double yn = over ? 0.0 : 1.0 - ftot;
-  const double y0n = y[(int64_t)51 * n + c];
-  dmax = fmax(dmax, fabs(yn - y0n));
-  double res = 0.0;
-  for (int e = 0; e < 3; ++e) {
-    // y1/ftot - y0 = (y1 - y0) - y1 (1 - 1/ftot) over species 1..51; N2 holds no C/H/O
-    const double dd = dm[e] - (over ? m1[e] * (1.0 - 1.0 / ftot) : 0.0) + th->mel[e][51] * (yn - y0n);
-    res = fmax(res, fabs(dd));
-  }
-  const double sum = over ? tot / ftot + stm / ftot : tot + stm + yn;
-  const double dT = (double)d(0), T = tk[c] * th->tstr + dT;
-  const bool in = flags_ok && stm <= th->stmf_max && stn <= th->stnd_max && T >= th->tmin && T <= th->tmax;
-  const bool slow = slow_bc && fabs(dT) <= tol_dt && dmax <= tol_dy;
-  const int why = !in ? 1 : (!(res <= tol_el) ? 2 : (!(fabs(sum - 1.0) <= tol_sum) ? 3 : (!slow ? 4 : 0)));
-  if (why) return why;
-  const int ip = T > 1000.0 ? 7 : 0;
-  double pp[6] = {0, 0, 0, 0, 0, 0}, hf = 0.0;
-#pragma unroll UC_GATE_UNROLL
-  for (int k = 0; k < NSPS; ++k) {
-    double y1 = yn;
-    if (k < 51) {
-      y1 = y1_of(k, y[(int64_t)k * n + c]);
-      if (over) y1 = y1 / ftot;
-    }
-    const double sw = y1 * th->rwm[k];
-    for (int e = 0; e < 6; ++e) pp[e] += th->nasah[k][ip + e] * sw;
-    hf += y1 * th->hfo[k] * th->hstr;
-    y[(int64_t)k * n + c] = y1;
-  }
-  const double H =
-      ((pp[0] + (pp[1] + (pp[2] + (pp[3] + pp[4] * T) * T) * T) * T) * T + pp[5]) * th->gasc / th->wmstr;
-  tk[c] = T / th->tstr;
-  ht[c] = (H - hf) / th->hstr;
-  return 0;
+// The gate runs four lanes per cell: lane l of a wave serves cell slot l % 16 and part
+// l / 16 of its species, k = part + 4 j (j = 0..12), whose Y0, predicted Y and tables it
+// holds in registers, so Y is read from HBM once and written once.  Sums over species
+// are each part's own in index order, then combined across the four lanes as
+// (p0 + p1) + (p2 + p3), which every lane of the cell evaluates to the same bits.
+constexpr int GPART = 4, GSP = NSPS / GPART;  // parts per cell, species per part
+constexpr int GATE_CELLS = GATE_BLOCK / 64 * 16;  // cells per block and round
+
+
+// per-species constants of the gate, staged in LDS: species k, network output 1 + k
+struct GateSp {
+  double mel[3], rwm, hfo, nasah[14], xlo, xhi, tlo, thi;
+  float tsd, tmean, tscale;
+};
+
+__device__ __forceinline__ double part_sum(double v) {
+  v += __shfl_xor(v, 16);
+  return v + __shfl_xor(v, 32);
+}
+__device__ __forceinline__ double part_max(double v) {
+  v = fmax(v, __shfl_xor(v, 16));
+  return fmax(v, __shfl_xor(v, 32));
+}
+__device__ __forceinline__ bool part_all(bool b) {
+  int v = b ? 1 : 0;
+  v &= __shfl_xor(v, 16);
+  v &= __shfl_xor(v, 32);
+  return v != 0;
 }
 
-// per-wave statistics of the gate: stats [0] accepted, [1] out of range, [2] residual,
-// [3] sum, [4] active (why = -1: no cell), [5..7] max |raw C/H/O residual| (bits of a
-// double); one atomic per wave and counter
-__device__ __forceinline__ void gate_stats(int why, const double* rr, unsigned long long* stats) {
+// Statistics of the gate: stats [0] accepted, [1] out of range, [2] residual, [3]
+// sum, [4] active, [5..7] max |raw C/H/O residual| (bits of a double; non-negative
+// doubles order as their bits).  A block sums its lanes' counts in LDS and adds to
+// stats once per counter, since same-address atomics from every wave serialise in L2.
+struct GateCount {
+  unsigned int n[5] = {0, 0, 0, 0, 0};
+  double rmax[3] = {0.0, 0.0, 0.0};
+  __device__ void add(int why, const double* rr) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) n[k] += why == k;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) rmax[e] = fmax(rmax[e], fabs(rr[e]));
+  }
+};
+__device__ __forceinline__ void gate_flush(const GateCount& gc, unsigned long long* s_st,
+                                           unsigned long long* __restrict__ stats) {
   const int l = threadIdx.x % 64;
+#pragma unroll
   for (int k = 0; k < 5; ++k) {
-    const unsigned long long m = __ballot(why == k);
-    if (l == 0 && m) atomicAdd(&stats[k], (unsigned long long)__popcll(m));
+    unsigned int v = gc.n[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (l == 0 && v) atomicAdd(&s_st[k], (unsigned long long)v);
   }
+#pragma unroll
   for (int e = 0; e < 3; ++e) {
-    double v = why >= 0 ? fabs(rr[e]) : 0.0;
+    double v = gc.rmax[e];
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-    if (l == 0 && v > 0.0) atomicMax(&stats[5 + e], (unsigned long long)__double_as_longlong(v));
+    if (l == 0 && v > 0.0) atomicMax(&s_st[5 + e], (unsigned long long)__double_as_longlong(v));
   }
+  __syncthreads();
+  if (threadIdx.x < 5 && s_st[threadIdx.x]) atomicAdd(&stats[threadIdx.x], s_st[threadIdx.x]);
+  if (threadIdx.x >= 5 && threadIdx.x < 8 && s_st[threadIdx.x]) atomicMax(&stats[threadIdx.x], s_st[threadIdx.x]);
 }
 
-// The gate as a kernel, one thread per listed cell, on the network's standardised
-// outputs: output j of list position q at out[q * sq + j * sj] (row-major [q][64] from
-// the separate-kernel network, species-major [64][ldo] from k_network).  pbuf: Y0^0.1
-// as [52][ldo] (k_network writes it; null: computed here, by the same function).
-// reject[c] = 1 where rejected, 0 where accepted (may be null); mask[c] = 10 where
-// accepted, iskip[c] where rejected (may be null): the ISKIP with which the FP64 step
-// then advances only the rejected cells.
-__global__ void __launch_bounds__(GATE_BLOCK) k_gate(const float* __restrict__ out, int64_t sq, int64_t sj,
+// The gate as a kernel, on the network's standardised outputs: output j of list
+// position q at out[q * sq + j * sj] (row-major [q][64] from the separate-kernel
+// network, species-major [64][ldo] from k_network).  pbuf: Y0^0.1 as [52][ldo]
+// (k_network writes it; null: computed here, by the same function).  reject[c] = 1
+// where rejected, 0 where accepted (may be null); mask[c] = 10 where accepted, iskip[c]
+// where rejected (may be null): the ISKIP with which the FP64 step then advances only
+// the rejected cells.  Every lane runs every shuffle, so the arithmetic of a cell is
+// done by all four of its lanes whether or not the cell is live or accepted; only the
+// stores are guarded.
+template <bool PBUF>
+__global__ void __launch_bounds__(GATE_BLOCK) __attribute__((amdgpu_waves_per_eu(GATE_WAVES))) k_gate(const float* __restrict__ out, int64_t sq, int64_t sj,
                                               const double* __restrict__ pbuf, int64_t ldo,
                                               const Norm* __restrict__ nm, const Thermo* __restrict__ th,
                                               double* __restrict__ y, double* __restrict__ tk,
@@ -309,32 +277,164 @@ __global__ void __launch_bounds__(GATE_BLOCK) k_gate(const float* __restrict__ o
                                               double tol_bc, int32_t* __restrict__ reject,
                                               const int32_t* __restrict__ iskip, int32_t* __restrict__ mask,
                                               unsigned long long* __restrict__ stats) {
-  __shared__ float sd[NT][GATE_BLOCK];  // the changes of this thread's cell, [output][thread]
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int why = -1;
-  double rr[3] = {0.0, 0.0, 0.0};
-  if (q < ncells) {
-    const int64_t c = cells[q];
-    auto pb = [&](int k) { return pbuf ? pbuf[(int64_t)k * ldo + q] : bc_base(y[(int64_t)k * n + c]); };
-    const double T0 = tk[c] * th->tstr;
-    bool ok = T0 >= th->xlo[0] && T0 <= th->xhi[0] && log_dt >= th->xlo[1] && log_dt <= th->xhi[1];
-    for (int k = 0; k < NSPS; ++k) {
-      const double v = (pb(k) - 1.0) / 0.1;
-      ok = ok && v >= th->xlo[k + 2] && v <= th->xhi[k + 2];
+  __shared__ GateSp sp[NSPS];
+  for (int k = threadIdx.x; k < NSPS; k += blockDim.x) {
+    GateSp& e = sp[k];
+    for (int i = 0; i < 3; ++i) e.mel[i] = th->mel[i][k];
+    e.rwm = th->rwm[k];
+    e.hfo = th->hfo[k];
+    for (int i = 0; i < 14; ++i) e.nasah[i] = th->nasah[k][i];
+    e.xlo = th->xlo[k + 2];
+    e.xhi = th->xhi[k + 2];
+    e.tlo = th->tlo[k + 1];
+    e.thi = th->thi[k + 1];
+    e.tsd = nm->tsd[k + 1];
+    e.tmean = nm->tmean[k + 1];
+    e.tscale = nm->tscale[k + 1];
+  }
+  __shared__ unsigned long long s_st[8];
+  if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
+  __syncthreads();
+  const int l = threadIdx.x % 64, part = l / 16;
+  GateCount gc;
+  const int64_t q0 = (int64_t)blockIdx.x * GATE_CELLS + (threadIdx.x / 64) * 16 + l % 16;
+  const bool live = q0 < ncells;
+  // a lane past the list repeats the last cell's arithmetic; its stores are skipped
+  const int64_t q = live ? q0 : ncells - 1, c = cells[q];
+  double y0[GSP], y1[GSP];
+  // features in the hull, outputs in range, Box-Cox changes slow (this part's species)
+  bool ok = true, slow = true;
+#pragma unroll
+  for (int j = 0; j < GSP; ++j) {
+    const int k = part + GPART * j;
+    const GateSp& e = sp[k];
+    y0[j] = y[(int64_t)k * n + c];
+    // Y0^0.1: from the network's features, or computed here by the same function
+    const double P = PBUF ? pbuf[(int64_t)k * ldo + q] : bc_base(y0[j]);
+    const double v = (P - 1.0) / 0.1;
+    ok = ok && v >= e.xlo && v <= e.xhi;
+    // output 1 + k: u = v tsd + tmean, in range; the change d = tscale sinh(u)
+    const float u = out[q * sq + (1 + k) * sj] * e.tsd + e.tmean;
+    ok = ok && (double)u >= e.tlo && (double)u <= e.thi;
+    const float d = e.tscale * sinhf(u);
+    slow = slow && fabs((double)d) <= tol_bc;
+    y1[j] = k < 51 ? y_after(P, d) : 0.0;
+  }
+  ok = part_all(ok);
+  slow = part_all(slow);
+  // dT (output 0) and the cell's own hull tests
+  const float u0 = out[q * sq] * nm->tsd[0] + nm->tmean[0];
+  const double dT = (double)(nm->tscale[0] * sinhf(u0));
+  const double T0 = tk[c] * th->tstr;
+  ok = ok && (double)u0 >= th->tlo[0] && (double)u0 <= th->thi[0] && T0 >= th->xlo[0] && T0 <= th->xhi[0] &&
+       log_dt >= th->xlo[1] && log_dt <= th->xhi[1];
+  // raw residual r = A dY and the weighted normal matrix M = A W A^T (symmetric 3x3),
+  // over species 1..49 (AR, HE and N2 hold no C, H or O)
+  double rr[3] = {0.0, 0.0, 0.0}, m[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < GSP; ++j) {
+    const int k = part + GPART * j;
+    if (k < 49) {
+      const GateSp& e = sp[k];
+      const double w = fmax(y0[j], y1[j]), a0 = e.mel[0], a1 = e.mel[1], a2 = e.mel[2];
+      rr[0] += a0 * (y1[j] - y0[j]);
+      rr[1] += a1 * (y1[j] - y0[j]);
+      rr[2] += a2 * (y1[j] - y0[j]);
+      m[0] += w * a0 * a0; m[1] += w * a0 * a1; m[2] += w * a0 * a2;
+      m[3] += w * a1 * a1; m[4] += w * a1 * a2; m[5] += w * a2 * a2;
     }
-    bool slow = true;
-    for (int j = 0; j < NT; ++j) {
-      bool r0, r1;
-      sd[j][threadIdx.x] = out_change(out[q * sq + j * sj], j, nm, th, tol_bc, r0, r1);
-      ok = ok && r0;
-      slow = slow && r1;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) rr[i] = part_sum(rr[i]);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) m[i] = part_sum(m[i]);
+  // lambda = M^-1 r (Cramer); the correction of species k is -w_k A_k^T lambda
+  const double c00 = m[3] * m[5] - m[4] * m[4], c01 = m[2] * m[4] - m[1] * m[5], c02 = m[1] * m[4] - m[2] * m[3];
+  const double c11 = m[0] * m[5] - m[2] * m[2], c12 = m[1] * m[2] - m[0] * m[4], c22 = m[0] * m[3] - m[1] * m[1];
+  const double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+  double lam[3] = {0.0, 0.0, 0.0};
+  if (det > 0.0) {
+    lam[0] = (c00 * rr[0] + c01 * rr[1] + c02 * rr[2]) / det;
+    lam[1] = (c01 * rr[0] + c11 * rr[1] + c12 * rr[2]) / det;
+    lam[2] = (c02 * rr[0] + c12 * rr[1] + c22 * rr[2]) / det;
+  }
+  // Y1 of species k < 51: predicted, corrected, clipped; its sum, C/H/O change and
+  // largest change
+  double tot = 0.0, dm[3] = {0.0, 0.0, 0.0}, m1[3] = {0.0, 0.0, 0.0}, dmax = 0.0;
+#pragma unroll
+  for (int j = 0; j < GSP; ++j) {
+    const int k = part + GPART * j;
+    if (k < 51) {
+      const GateSp& e = sp[k];
+      double v = y1[j];
+      if (k < 49) v -= fmax(y0[j], v) * (e.mel[0] * lam[0] + e.mel[1] * lam[1] + e.mel[2] * lam[2]);
+      v = v < 0.0 ? 0.0 : v;
+      y1[j] = v;
+      tot += v;
+      dmax = fmax(dmax, fabs(v - y0[j]));
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        dm[i] += e.mel[i] * (v - y0[j]);
+        m1[i] += e.mel[i] * v;
+      }
     }
-    why = gate_cell([&](int j) { return sd[j][threadIdx.x]; }, pb, th, y, tk, ht, stmf[c], stnd[c], n, c, ok, slow,
-                    tol_el, tol_sum, tol_dt, tol_dy, rr);
+  }
+  tot = part_sum(tot);
+  dmax = part_max(dmax);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    dm[i] = part_sum(dm[i]);
+    m1[i] = part_sum(m1[i]);
+  }
+  // N2 closure as SOLVESP does (solvesp.f:2839-2914); N2 is species 51 = part 3, j = 12
+  const double stm = stmf[c], stn = stnd[c];
+  const double ftot = tot + stm;
+  const bool over = ftot > 1.0;
+  const double yn = over ? 0.0 : 1.0 - ftot;
+  const double y0n = __shfl(y0[GSP - 1], 48 + l % 16);
+  dmax = fmax(dmax, fabs(yn - y0n));
+  double res = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    // y1/ftot - y0 = (y1 - y0) - y1 (1 - 1/ftot) over species 1..51; N2 holds no C/H/O
+    const double dd = dm[i] - (over ? m1[i] * (1.0 - 1.0 / ftot) : 0.0) + sp[51].mel[i] * (yn - y0n);
+    res = fmax(res, fabs(dd));
+  }
+  const double sum = over ? tot / ftot + stm / ftot : tot + stm + yn;
+  const double T = T0 + dT;
+  const bool in = ok && stm <= th->stmf_max && stn <= th->stnd_max && T >= th->tmin && T <= th->tmax;
+  const bool fast = !(slow && fabs(dT) <= tol_dt && dmax <= tol_dy);
+  const int why = !live ? -1 : (!in ? 1 : (!(res <= tol_el) ? 2 : (!(fabs(sum - 1.0) <= tol_sum) ? 3 : (fast ? 4 : 0))));
+  // the applied state: Y (N2 closed, renormalised when over), and HT, the sensible
+  // enthalpy of the new T and Y, whose inverse is solveh.f:1170-1230
+  const int ip = T > 1000.0 ? 7 : 0;
+  double pp[6] = {0, 0, 0, 0, 0, 0}, hf = 0.0;
+#pragma unroll
+  for (int j = 0; j < GSP; ++j) {
+    const int k = part + GPART * j;
+    const GateSp& e = sp[k];
+    const double v = k < 51 ? (over ? y1[j] / ftot : y1[j]) : yn;
+    const double sw = v * e.rwm;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) pp[i] += e.nasah[ip + i] * sw;
+    hf += v * e.hfo * th->hstr;
+    if (why == 0) y[(int64_t)k * n + c] = v;
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) pp[i] = part_sum(pp[i]);
+  hf = part_sum(hf);
+  const double H =
+      ((pp[0] + (pp[1] + (pp[2] + (pp[3] + pp[4] * T) * T) * T) * T) * T + pp[5]) * th->gasc / th->wmstr;
+  if (part == 0 && why == 0) {
+    tk[c] = T / th->tstr;
+    ht[c] = (H - hf) / th->hstr;
+  }
+  if (part == 0 && live) {
     if (reject) reject[c] = why != 0;
     if (mask) mask[c] = why == 0 ? 10 : iskip[c];
+    gc.add(why, rr);
   }
-  gate_stats(why, rr, stats);
+  gate_flush(gc, s_st, stats);
 }
 
 // ---- the network on a field: features -> MLP -> outputs, 16 cells per wave, one launch
@@ -370,8 +470,11 @@ struct InferArgs {
 template <int NTILES, int KSTEPS>
 __device__ __forceinline__ void infer_layer(const bf16_t* __restrict__ w, const bf16x8* in, f32x4* acc, int cl,
                                             int g, bf16_t* __restrict__ slab) {
-  constexpr int K = KSTEPS * 32, ROWS = NTILES * 16, CHUNKS = ROWS * 4 / 256;  // 16-byte chunks a thread
-  constexpr int BUF = 256 * 32;                                                   // one buffer, bf16 elements
+  // 16-byte chunks of a k-step's slab, and how many each thread stages (the last round
+  // partial when the block has more threads than the slab has chunks)
+  constexpr int K = KSTEPS * 32, ROWS = NTILES * 16, NCH = ROWS * 4, NTH = 64 * NET_WAVES;
+  constexpr int CHUNKS = (NCH + NTH - 1) / NTH;
+  constexpr int BUF = 256 * 32;  // one buffer, bf16 elements
   const int tid = threadIdx.x;
 #pragma unroll
   for (int t = 0; t < NTILES; ++t)
@@ -380,11 +483,14 @@ __device__ __forceinline__ void infer_layer(const bf16_t* __restrict__ w, const 
   bf16x8 ld[CHUNKS];
 #pragma unroll
   for (int i = 0; i < CHUNKS; ++i) {
-    const int ch = tid + 256 * i, row = ch / 4, col = (ch % 4) * 8;
-    ld[i] = *(const bf16x8*)(w + row * K + col);
+    const int ch = tid + NTH * i, row = ch / 4, col = (ch % 4) * 8;
+    if (NCH % NTH == 0 || ch < NCH) ld[i] = *(const bf16x8*)(w + row * K + col);
   }
 #pragma unroll
-  for (int i = 0; i < CHUNKS; ++i) *(bf16x8*)(slab + (tid + 256 * i) * 8) = ld[i];
+  for (int i = 0; i < CHUNKS; ++i) {
+    const int ch = tid + NTH * i;
+    if (NCH % NTH == 0 || ch < NCH) *(bf16x8*)(slab + ch * 8) = ld[i];
+  }
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < KSTEPS; ++s) {
@@ -392,8 +498,8 @@ __device__ __forceinline__ void infer_layer(const bf16_t* __restrict__ w, const 
     if (s + 1 < KSTEPS)
 #pragma unroll
       for (int i = 0; i < CHUNKS; ++i) {
-        const int ch = tid + 256 * i, row = ch / 4, col = (ch % 4) * 8;
-        ld[i] = *(const bf16x8*)(w + row * K + 32 * (s + 1) + col);
+        const int ch = tid + NTH * i, row = ch / 4, col = (ch % 4) * 8;
+        if (NCH % NTH == 0 || ch < NCH) ld[i] = *(const bf16x8*)(w + row * K + 32 * (s + 1) + col);
       }
 #pragma unroll
     for (int t = 0; t < NTILES; ++t) {
@@ -402,7 +508,10 @@ __device__ __forceinline__ void infer_layer(const bf16_t* __restrict__ w, const 
     }
     if (s + 1 < KSTEPS)
 #pragma unroll
-      for (int i = 0; i < CHUNKS; ++i) *(bf16x8*)(slab + ((s + 1) % 2) * BUF + (tid + 256 * i) * 8) = ld[i];
+      for (int i = 0; i < CHUNKS; ++i) {
+        const int ch = tid + NTH * i;
+        if (NCH % NTH == 0 || ch < NCH) *(bf16x8*)(slab + ((s + 1) % 2) * BUF + ch * 8) = ld[i];
+      }
     __syncthreads();
   }
 }
@@ -417,10 +526,10 @@ __device__ __forceinline__ void infer_activate(const f32x4* acc, const float* __
     }
 }
 
-__global__ void __launch_bounds__(256) k_network(InferArgs a) {
+__global__ void __launch_bounds__(64 * NET_WAVES) k_network(InferArgs a) {
   __shared__ __attribute__((aligned(16))) bf16_t slab[2 * 256 * 32];  // weight k-steps, double-buffered
   const int wv = threadIdx.x / 64, l = threadIdx.x % 64, g = l / 16, cl = l % 16;
-  const int64_t q = ((int64_t)blockIdx.x * 4 + wv) * 16 + cl;
+  const int64_t q = ((int64_t)blockIdx.x * NET_WAVES + wv) * 16 + cl;
   const bool live = q < a.ncells;
   const int64_t c = live ? a.cells[q] : 0;
   const Norm* nm = a.nm;
@@ -529,7 +638,7 @@ int us_gate(const float* out, const void* norm, const void* thermo, double* y, d
             double tol_el, double tol_sum, double tol_dt, double tol_dy, double tol_bc, int32_t* reject,
             unsigned long long* stats, void* stream) {
   if (ncells <= 0) return 0;
-  k_gate<<<blocks(ncells, GATE_BLOCK), GATE_BLOCK, 0, (hipStream_t)stream>>>(out, NPAD, 1, nullptr, 0, (const Norm*)norm,
+  k_gate<false><<<blocks(ncells, GATE_CELLS), GATE_BLOCK, 0, (hipStream_t)stream>>>(out, NPAD, 1, nullptr, 0, (const Norm*)norm,
                                                                 (const Thermo*)thermo, y, tk, ht, stmf, stnd, n, cells,
                                                                 ncells, log_dt, tol_el, tol_sum, tol_dt, tol_dy, tol_bc,
                                                                 reject, nullptr, nullptr, stats);
@@ -557,8 +666,8 @@ int us_infer(const uint16_t* const* w, const float* const* b, const void* norm, 
   a.y = y; a.tk = tk; a.out = out; a.pbuf = pbuf; a.ldo = ldo; a.cells = cells; a.n = n; a.ncells = ncells;
   a.tstr = tstr; a.log_dt = log_dt;
   hipStream_t s = (hipStream_t)stream;
-  k_network<<<blocks(ncells, 64), 256, 0, s>>>(a);
-  k_gate<<<blocks(ncells, GATE_BLOCK), GATE_BLOCK, 0, s>>>(out, 1, ldo, pbuf, ldo, (const Norm*)norm, (const Thermo*)thermo, y, tk,
+  k_network<<<blocks(ncells, 16 * NET_WAVES), 64 * NET_WAVES, 0, s>>>(a);
+  k_gate<true><<<blocks(ncells, GATE_CELLS), GATE_BLOCK, 0, s>>>(out, 1, ldo, pbuf, ldo, (const Norm*)norm, (const Thermo*)thermo, y, tk,
                                               ht, stmf, stnd, n, cells, ncells, log_dt, tol_el, tol_sum, tol_dt,
                                               tol_dy, tol_bc, nullptr, iskip, mask, stats);
   HIPCHK(hipGetLastError());
