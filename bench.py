@@ -187,6 +187,37 @@ def coupled_cpu_baseline(nstar, nh):
             "s_per_cell_sweep": r.t_chem / (cells * nstar), "cores": 1}
 
 
+# SPGSOLV's coefficient assembly (uc_assemble): algorithmic HBM bytes per cell.  k_asm_ekt
+# reads TK, VTC, TMU and 52 Y and writes EKT (56 doubles); k_asm_coef reads RHO, U, V, TMU,
+# Y (radius), EKT, STDF and 51 VDF and writes 212 coefficients (270 doubles); neighbours'
+# values are counted once, as a cell's own.
+ASM_BYTES_PER_CELL = 8 * (56 + 270)
+
+
+def assemble_bench(ctx, dev, ctl, f0, reps=20):
+    """Time uc_assemble on the field (synthetic_transport seed 3) with HIP events on the
+    stream it runs on; the roofline is HBM (the assembly is per-face memory work)."""
+    import torch
+
+    from unicorn_amd.chem import DeviceTransport
+    from unicorn_amd.fields import synthetic_transport
+    dtr = DeviceTransport(synthetic_transport(f0, seed=3))
+    s = torch.cuda.current_stream()
+    ctx.assemble(dev, ctl, dtr)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        ctx.assemble(dev, ctl, dtr)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    achieved = ASM_BYTES_PER_CELL * f0.n / (ms * 1e-3)
+    return {"kernel": "k_asm_ekt + k_asm_coef (uc_assemble)", "ms_per_call": ms, "bound": "hbm",
+            "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+            "bytes_per_cell": ASM_BYTES_PER_CELL,
+            "note": "SPGSOLV's E/W/P/Q, STE..STS, ZE..ZS and EKT for every cell; equal bit for bit to spgsolv.f"}
+
+
 def coupled_bench(args, f0=None, cpu=True):
     """The field with synthetic transport coefficients through uc_step_coupled."""
     import torch
@@ -237,6 +268,7 @@ def coupled_bench(args, f0=None, cpu=True):
                          "wavefront_steps_per_step": wave_steps / steps, "diagonals": ndiag},
            "sweeps": {"global_species": [s.isor1 for s in stats], "global_enthalpy": [s.isor2 for s in stats],
                       "first_segment": [s.first_segment for s in stats]}}
+    out["assemble"] = assemble_bench(ctx, dev, ctl, f0)
     if cpu and not args.no_cpu:
         nstar = int(np.median([min(s.isor1, 100) for s in stats]))
         nh = int(np.median([min(s.isor2, 100) for s in stats]))

@@ -132,6 +132,22 @@ typedef struct {
     int32_t isym;                /* /CB01/ ISYM */
 } uc_coupling;
 
+/* SPGSOLV's inputs to its coefficient assembly beyond the state (spgsolv.f:98-349);
+ * device pointers in the field layout.  U(J,I) lies on the face between I-1 and I,
+ * V(J,I) on the face between J-1 and J (/CB04/).  sigsp and sigh are SPGSOLV's SIGSP
+ * and SIGH arguments (main:346-347, both 0.9). */
+typedef struct {
+    const double *vtc, *tmu;     /* [n] /CB07/ VTC, TMU */
+    const double *vdf;           /* [52][n] /CB07/ VDF (species 1..51 used) */
+    const double *stdf;          /* [n] /SOOT/ STDF */
+    const double *u, *v;         /* [n] /CB04/ U, V */
+    const double *ygeo;          /* [n] /CB03/ Y */
+    const double *xxc, *xxs;     /* [li] */
+    const double *yyc, *yys;     /* [lj] */
+    double sigsp, sigh;
+    int32_t isym;                /* /CB01/ ISYM */
+} uc_transport;
+
 typedef struct uc_context uc_context;
 
 int uc_abi_version(void);
@@ -170,6 +186,16 @@ int uc_step(uc_context *ctx, const uc_field *field, const uc_controls *ctl, void
  * enthalpy flux to RHSZ (solveh.f:59-115).  Global stops only (UC_MODE_GLOBAL); one grid. */
 int uc_step_coupled(uc_context *ctx, const uc_field *field, const uc_controls *ctl,
                     const uc_coupling *coupling, void *stream, uc_stats *stats);
+
+/* SPGSOLV's coefficient assembly (spgsolv.f:98-1290; replaces that part of SUBROUTINE
+ * SPGSOLV, spgsolv.f:6-1290, whose outputs are the /DUMMY/, /DUM0/, /DUMX/, /DUMS/ and
+ * /DUMY/ EKT arrays): EKT of every cell into ekt[n], and for the interior cells with
+ * ISKIP < 10 the E/W/P/Q coefficients of species 1..51 into coef[204][n] (uc_coupling's
+ * order), STE..STS into stc[4][n] and ZE..ZS into zc[4][n]; other cells are left as they
+ * were.  Reads field->rho, y, tk, iskip and ctl->dt.  Equal bit for bit to the reference
+ * routine.  Asynchronous on `stream`; allocates nothing. */
+int uc_assemble(uc_context *ctx, const uc_field *field, const uc_controls *ctl, const uc_transport *tr,
+                double *coef, double *stc, double *zc, double *ekt, void *stream);
 
 /* Same, on host arrays (copies in and out; the PCIe cost is in ms_total). */
 int uc_step_host(uc_context *ctx, const uc_field *host_field, const uc_controls *ctl,

@@ -157,3 +157,25 @@ def scales(ref=None):
     r = np.array(ref or default_ref(), dtype=np.float64)
     lib().uco_scales(ptr(r), ptr(out))
     return out
+
+
+def assemble(field, tr, ref=None) -> dict:
+    """SPGSOLV's coefficient assembly (oracle/chem_oracle.c uco_assemble) on a field and
+    unicorn_amd.fields.Transport inputs: ekt (n,), coef (204, n), stc (4, n), zc (4, n).
+    Cells the assembly does not write (boundary, ISKIP >= 10) are NaN."""
+    L = lib()
+    n = field.n
+    r = np.asarray(default_ref() if ref is None else ref, dtype=np.float64)
+    out = {"ekt": np.full(n, np.nan), "coef": np.full((204, n), np.nan), "stc": np.full((4, n), np.nan),
+           "zc": np.full((4, n), np.nan)}
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    ins = [c(field.rho), c(field.y), c(field.tk)]
+    isk = np.ascontiguousarray(field.iskip, dtype=np.int32)
+    trs = [c(a) for a in (tr.vtc, tr.tmu, tr.vdf, tr.stdf, tr.u, tr.v, tr.ygeo, tr.xxc, tr.xxs, tr.yyc, tr.yys)]
+    L.uco_assemble.restype = C.c_int
+    rc = L.uco_assemble(ptr(r), C.c_int(field.li), C.c_int(field.lj), *[ptr(a) for a in ins], ptr(isk, C.c_int32),
+                        *[ptr(a) for a in trs], C.c_double(field.dt), C.c_double(tr.sigsp), C.c_double(tr.sigh),
+                        C.c_int32(tr.isym), ptr(out["coef"]), ptr(out["stc"]), ptr(out["zc"]), ptr(out["ekt"]))
+    if rc != 0:
+        raise RuntimeError(f"uco_assemble returned {rc}")
+    return out

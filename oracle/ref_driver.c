@@ -40,6 +40,10 @@
 extern void ckrates_(void);
 extern void solvesp_(int32_t *isor1, double *relxsp, double *tolrsp, double *spnorm);
 extern void solveh_(int32_t *isor2, double *relxh, double *tolrh, double *sigsp, double *hnorm);
+/* present only in ref_driver_asm (the flang-built spgsolv.f, which INCLUDEs the three above) */
+extern void spgsolv_(int32_t *isor1, double *relxsp, double *tolrsp, double *resdc, int32_t *isor2,
+                     double *relxh, double *tolrh, double *resdh, double *sigh, double *sigsp)
+    __attribute__((weak));
 
 _Static_assert(sizeof(cb01_) == 0x5af70, "cb01");
 _Static_assert(sizeof(cb03_) == 0x16f1f8, "cb03");
@@ -186,8 +190,92 @@ static double now(void) {
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
+/* SPGSOLV's coefficient assembly alone (ICHEM = ITHRM = 0: spgsolv.f:98-1290, then its
+ * epilogue, which with SA = FA changes nothing read here).  Input 'UCA1': nj, ni, isym
+ * (int64); dt, sigsp, sigh, ref_values[6]; rho[n], y[52][n], tk[n], iskip[n] (int32),
+ * vtc[n], tmu[n], vdf[52][n], stdf[n], u[n], v[n], ygeo[n], xxc[ni], xxs[ni], yyc[nj],
+ * yys[nj] (field layout, j fastest).  Output: ekt[n], emu[n], coef[204][n] (/DUMMY/ then
+ * /DUM0/), stc[4][n] (STE..STS), zc[4][n] (ZE..ZS). */
+static int run_assemble(const char *inp, const char *outp) {
+    if (!spgsolv_) die("this driver has no SPGSOLV (build ref_driver_asm)");
+    FILE *f = fopen(inp, "rb");
+    if (!f) die("cannot open input");
+    int64_t hdr[4];
+    rd(hdr, sizeof hdr, f);
+    if (hdr[0] != 0x31414355) die("bad magic"); /* 'UCA1' */
+    const int nj = (int)hdr[1], ni = (int)hdr[2];
+    if (nj > LJ || ni > LI || nj < 3 || ni < 3) die("field does not fit the 711x131 reference grid");
+    double sc[9];
+    rd(sc, sizeof sc, f);
+    const size_t n = (size_t)nj * ni;
+    double *rho = malloc(8 * n), *y = malloc(8 * n * LSP), *tk = malloc(8 * n), *vtc = malloc(8 * n),
+           *tmu = malloc(8 * n), *vdf = malloc(8 * n * LSP), *stdf = malloc(8 * n), *u = malloc(8 * n),
+           *v = malloc(8 * n), *yg = malloc(8 * n), *xxc = malloc(8 * ni), *xxs = malloc(8 * ni),
+           *yyc = malloc(8 * nj), *yys = malloc(8 * nj);
+    int32_t *isk = malloc(4 * n);
+    rd(rho, 8 * n, f); rd(y, 8 * n * LSP, f); rd(tk, 8 * n, f); rd(isk, 4 * n, f);
+    rd(vtc, 8 * n, f); rd(tmu, 8 * n, f); rd(vdf, 8 * n * LSP, f); rd(stdf, 8 * n, f);
+    rd(u, 8 * n, f); rd(v, 8 * n, f); rd(yg, 8 * n, f);
+    rd(xxc, 8 * ni, f); rd(xxs, 8 * ni, f); rd(yyc, 8 * nj, f); rd(yys, 8 * nj, f);
+    fclose(f);
+    setup(sc + 3);
+    /* benign state on the whole 711 x 131 grid, then the field in its corner */
+    for (int c = 0; c < LE; ++c) {
+        cb01_.iskip[c] = 10;
+        for (int k = 0; k < LSP; ++k) { cb04_.fsp[k][c] = 0.0; cb07_.vdf[k][c] = 0.0; }
+        cb04_.fsp[1][c] = 0.233; cb04_.fsp[LSP - 1][c] = 0.767;
+        cb04_.tk[c] = 300.0 / S.tstr; cb04_.rho[c] = 1.0; cb04_.ht[c] = 0.0;
+        cb04_.u[c] = 0.0; cb04_.v[c] = 0.0;
+        cb07_.vtc[c] = 0.0; cb07_.tmu[c] = 0.0; soot_.stdf[c] = 0.0;
+        cb03_.y[c] = 1.0;
+    }
+    for (int i = 0; i < LI; ++i) { cb03_.xxc[i] = 1.0; cb03_.xxs[i] = 1.0; }
+    for (int j = 0; j < LJ; ++j) { cb03_.yyc[j] = 1.0; cb03_.yys[j] = 1.0; }
+    for (int i = 0; i < ni; ++i) { cb03_.xxc[i] = xxc[i]; cb03_.xxs[i] = xxs[i]; }
+    for (int j = 0; j < nj; ++j) { cb03_.yyc[j] = yyc[j]; cb03_.yys[j] = yys[j]; }
+    for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < nj; ++j) {
+            const size_t q = (size_t)i * nj + j;
+            const int c = fidx(j, i);
+            cb01_.iskip[c] = isk[q];
+            cb04_.rho[c] = rho[q]; cb04_.tk[c] = tk[q]; cb04_.u[c] = u[q]; cb04_.v[c] = v[q];
+            for (int k = 0; k < LSP; ++k) {
+                cb04_.fsp[k][c] = y[(size_t)k * n + q];
+                cb07_.vdf[k][c] = vdf[(size_t)k * n + q];
+            }
+            cb07_.vtc[c] = vtc[q]; cb07_.tmu[c] = tmu[q]; soot_.stdf[c] = stdf[q];
+            cb03_.y[c] = yg[q];
+        }
+    cb03_.dt = sc[0];
+    cb01_.isym = (int32_t)hdr[3];
+    cb01_.ichem = 0;
+    cb01_.ithrm = 0;
+    int32_t isor1 = 1, isor2 = 1;
+    double relxsp = 1.0, tolrsp = 1.0, resdc = 0.0, relxh = 1.0, tolrh = 1.0, resdh = 0.0;
+    double sigsp = sc[1], sigh = sc[2];
+    spgsolv_(&isor1, &relxsp, &tolrsp, &resdc, &isor2, &relxh, &tolrh, &resdh, &sigh, &sigsp);
+    FILE *fo = fopen(outp, "wb");
+    if (!fo) die("cannot open output");
+    double *buf = malloc(8 * n);
+#define DUMPA(expr)                                                                  \
+    do {                                                                             \
+        for (int i = 0; i < ni; ++i)                                                 \
+            for (int j = 0; j < nj; ++j) { int c = fidx(j, i); buf[(size_t)i * nj + j] = (expr); } \
+        wr(buf, 8 * n, fo);                                                          \
+    } while (0)
+    DUMPA(dumy_.ekt[c]);
+    DUMPA(dumy_.emu[c]);
+    for (int g = 0; g < 204; ++g) DUMPA(g < 55 ? dummy_.a[g][c] : dum0_.a[g - 55][c]);
+    DUMPA(dums_.ste[c]); DUMPA(dums_.stw[c]); DUMPA(dums_.stn[c]); DUMPA(dums_.sts[c]);
+    DUMPA(dumx_.ze[c]); DUMPA(dumx_.zw[c]); DUMPA(dumx_.zn[c]); DUMPA(dumx_.zs[c]);
+#undef DUMPA
+    fclose(fo);
+    return 0;
+}
+
 int main(int argc, char **argv) {
-    if (argc < 3) die("usage: ref_driver mech <out> | run <in> <out>");
+    if (argc < 3) die("usage: ref_driver mech <out> | run <in> <out> | asm <in> <out>");
+    if (!strcmp(argv[1], "asm") && argc >= 4) return run_assemble(argv[2], argv[3]);
     if (!strcmp(argv[1], "mech")) {
         double rv[6] = {10.0, 294.0, 1.0133e+05, 0.0, 0.100, 0.233};
         setup(rv);

@@ -13,6 +13,9 @@ Cases:
   coupled 25x17 CH4 field (seed 13), soot in 30 % of cells, a block of ISKIP=10 cells
           inside, and the coupling inputs of unicorn_amd.fields.synthetic_coupling (seed 17): E/W/P/Q, STE..STS, ZE..ZS,
           RDHSP inputs, geometry and velocities (stored as cpl_*): 1 and 3 steps
+  assemble the coupled case's field with unicorn_amd.fields.synthetic_transport (seed 19):
+          SPGSOLV's coefficient assembly by the flang-built spgsolv.f (ref_driver_asm),
+          inputs tr_*, outputs asm_* (EKT, EMU, E/W/P/Q, STE..STS, ZE..ZS)
 Each holds the input field and, per step count, the final field, QDOT, QDSP of
 24 interior cells, the sweep counts and residuals per step; plus RC/SOUR of
 those 24 cells after the first CKRATES.
@@ -28,7 +31,7 @@ ROOT = Path(__file__).resolve().parents[2]
 sys.path.insert(0, str(ROOT))
 
 from oracle import refrun  # noqa: E402
-from unicorn_amd.fields import flame_field, synthetic_coupling  # noqa: E402
+from unicorn_amd.fields import flame_field, synthetic_coupling, synthetic_transport  # noqa: E402
 
 CPL_ARRAYS = ("coef", "stc", "zc", "ekt", "tmu", "vdf", "ygeo", "xxc", "xxs", "yyc", "yys", "u", "v", "w")
 
@@ -89,7 +92,26 @@ CASE_SPECS = {
     "c7band": lambda: case("c7band", flame_field(41, 25, "C7H16", 7, preignition_band=0.1), [2]),
     "sooty": lambda: case("sooty", sooty_field(), [1]),
     "coupled": lambda: (lambda h: case("coupled", h, [1, 3], synthetic_coupling(h, seed=17)))(coupled_field()),
+    "assemble": lambda: assemble_case(),
 }
+
+ASM_IN = ("vtc", "tmu", "vdf", "stdf", "u", "v", "ygeo", "xxc", "xxs", "yyc", "yys")
+
+
+def assemble_case():
+    """SPGSOLV's coefficient assembly (oracle/_ref/ref_driver_asm, the flang-built
+    spgsolv.f) on the coupled case's field with synthetic transport inputs (seed 19):
+    the field, the inputs (tr_*) and the outputs (asm_*)."""
+    h = coupled_field()
+    tr = synthetic_transport(h, seed=19)
+    out = refrun.assemble(h, tr)
+    arrays = dict(li=h.li, lj=h.lj, in_rho=h.rho, in_y=h.y, in_tk=h.tk, in_ht=h.ht, in_stmf=h.stmf,
+                  in_stnd=h.stnd, in_iskip=h.iskip, dt=h.dt, fuel=h.fuel, tr_sigsp=tr.sigsp, tr_sigh=tr.sigh,
+                  tr_isym=tr.isym)
+    arrays.update({f"tr_{k}": getattr(tr, k) for k in ASM_IN})
+    arrays.update({f"asm_{k}": v for k, v in out.items()})
+    np.savez_compressed(OUT / "assemble.npz", **arrays)
+    print("assemble: written", flush=True)
 
 
 def main(only=()):

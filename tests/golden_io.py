@@ -20,6 +20,22 @@ def load(name: str):
     return f, z
 
 
+ASM_IN = ("vtc", "tmu", "vdf", "stdf", "u", "v", "ygeo", "xxc", "xxs", "yyc", "yys")
+
+
+def load_transport(z):
+    """The transport inputs of the assembly fixture (unicorn_amd.fields.Transport)."""
+    from unicorn_amd.fields import Transport
+    return Transport(*[z[f"tr_{k}"].copy() for k in ASM_IN], sigsp=float(z["tr_sigsp"]), sigh=float(z["tr_sigh"]),
+                     isym=int(z["tr_isym"]))
+
+
+def assembled_cells(f):
+    """The cells SPGSOLV's assembly writes: interior, ISKIP < 10 (spgsolv.f:1115-1117)."""
+    inter = f.interior()
+    return inter[f.iskip[inter] < 10]
+
+
 def load_coupling(z):
     """The coupling inputs stored with a coupled fixture (unicorn_amd.fields.Coupling)."""
     from unicorn_amd.fields import Coupling

@@ -61,9 +61,15 @@ class UcCoupling(C.Structure):
         ("sigsp", C.c_double), ("isym", C.c_int32)]
 
 
+class UcTransport(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "vtc", "tmu", "vdf", "stdf", "u", "v", "ygeo", "xxc", "xxs", "yyc", "yys")] + [
+        ("sigsp", C.c_double), ("sigh", C.c_double), ("isym", C.c_int32)]
+
+
 EXPORTS = ("uc_abi_version", "uc_last_error", "uc_scales_from_card", "uc_create", "uc_create_batch",
            "uc_destroy", "uc_step", "uc_step_host", "uc_step_batch", "uc_autotune", "uc_set_tuning",
-           "uc_get_tuning", "uc_cells_changed", "uc_clip", "uc_set_clip", "uc_step_coupled")
+           "uc_get_tuning", "uc_cells_changed", "uc_clip", "uc_set_clip", "uc_step_coupled", "uc_assemble")
 
 _lib = None
 
@@ -106,6 +112,8 @@ def library() -> C.CDLL:
                                  C.POINTER(UcStats)]
     lib.uc_step_coupled.argtypes = [C.c_void_p, C.POINTER(UcField), C.POINTER(UcControls), C.POINTER(UcCoupling),
                                     C.c_void_p, C.POINTER(UcStats)]
+    lib.uc_assemble.argtypes = [C.c_void_p, C.POINTER(UcField), C.POINTER(UcControls), C.POINTER(UcTransport),
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     _lib = lib
     return lib
 
@@ -209,6 +217,25 @@ class DeviceCoupling:
         return s
 
 
+class DeviceTransport:
+    """A unicorn_amd.fields.Transport resident in device memory (torch tensors)."""
+
+    NAMES = ("vtc", "tmu", "vdf", "stdf", "u", "v", "ygeo", "xxc", "xxs", "yyc", "yys")
+
+    def __init__(self, tr, device="cuda"):
+        import torch
+        self.sigsp, self.sigh, self.isym = tr.sigsp, tr.sigh, tr.isym
+        for n in self.NAMES:
+            setattr(self, n, torch.from_numpy(np.ascontiguousarray(getattr(tr, n), dtype=np.float64)).to(device))
+
+    def c(self) -> UcTransport:
+        s = UcTransport()
+        for n in self.NAMES:
+            setattr(s, n, getattr(self, n).data_ptr())
+        s.sigsp, s.sigh, s.isym = self.sigsp, self.sigh, self.isym
+        return s
+
+
 def clip(dev: "DeviceField"):
     """main:821-828 on a device field (Y<0 -> 0, N2 by closure, every cell), on the
     current stream: the first phase of a step, on its own."""
@@ -260,6 +287,29 @@ class ChemStep:
         _check(self.lib.uc_step_coupled(self.h, C.byref(fc), C.byref(cc), C.byref(pc), C.c_void_p(stream),
                                         C.byref(st)))
         return StepStats.of(st)
+
+    def assemble(self, f: DeviceField, ctl: Controls, tr: "DeviceTransport", stream=None) -> "DeviceCoupling":
+        """SPGSOLV's coefficient assembly on the device (uc_assemble): the coupling inputs of
+        step_coupled (coefficients, EKT; TMU, VDF, geometry and velocities from tr), in
+        new device arrays.  Cells the assembly does not write (boundary, ISKIP >= 10)
+        hold zero coefficients."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        n = f.rho.numel()
+        cp = DeviceCoupling.__new__(DeviceCoupling)
+        for name in DeviceCoupling.NAMES:
+            setattr(cp, name, None)
+        z = lambda *shape: torch.zeros(shape, dtype=torch.float64, device=f.rho.device)
+        cp.coef, cp.stc, cp.zc, cp.ekt = z(204, n), z(4, n), z(4, n), z(n)
+        for name in ("tmu", "vdf", "ygeo", "xxc", "xxs", "yyc", "yys", "u", "v"):
+            setattr(cp, name, getattr(tr, name))
+        cp.w = z(n)
+        cp.sigsp, cp.isym = tr.sigsp, tr.isym
+        fc, cc, tc = f.c(), ctl.c(), tr.c()
+        _check(self.lib.uc_assemble(self.h, C.byref(fc), C.byref(cc), C.byref(tc), cp.coef.data_ptr(),
+                                    cp.stc.data_ptr(), cp.zc.data_ptr(), cp.ekt.data_ptr(), C.c_void_p(stream)))
+        return cp
 
     def step_batch(self, fields: list[DeviceField], ctls: list[Controls], stream=None) -> list[StepStats]:
         """Advance several device-resident grids by one chemistry step each, in one batch."""

@@ -1104,6 +1104,109 @@ void fill_params(const uc_scales& sc, UcParams& p);
 
 #include "chem_host.inc"
 
+
+// ---- SPGSOLV's coefficient assembly (spgsolv.f:98-1290), the input side of the coupled
+// step.  FP64 with contraction off, so every value is the reference's operations in its
+// order and equals the flang-built routine bit for bit.
+struct AsmArgs {
+  const double *rho, *y, *tk;
+  const int32_t* iskip;
+  const double *vtc, *tmu, *vdf, *stdf, *u, *v, *ygeo, *xxc, *xxs, *yyc, *yys;
+  double *coef, *stc, *zc, *ekt;
+  int64_t n;
+  int32_t li, lj;
+  double dt, sigsp, sigh, sym, rei, beta2, beta3, tstr, gasc, wmstr, hstr;
+};
+
+// EKT of every cell (spgsolv.f:102-176): cp of the mixture from POLSP (P->nasa), then
+// EKT = REI*(BETA2*VTC/CP + TMU/SIGH)
+__global__ void __launch_bounds__(256) k_asm_ekt(const UcParams* __restrict__ P, AsmArgs a) {
+#pragma clang fp contract(off)
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.n) return;
+  const double tkd = a.tk[c] * a.tstr;
+  const double tkd2 = tkd * tkd, tkd3 = tkd * tkd2, tkd4 = tkd * tkd3;
+  const int ip = tkd > 1000.0 ? 7 : 0;  // /CB08/ TPOL1
+  double tcp = 0.0;
+#pragma unroll 4
+  for (int k = 0; k < NSP; ++k) {
+    const double* p = &P->nasa[k][ip];
+    const double cpk = (p[0] + p[1] * tkd + p[2] * tkd2 + p[3] * tkd3 + p[4] * tkd4) / P->wm[k];
+    tcp = tcp + cpk * a.y[(int64_t)k * a.n + c];
+  }
+  tcp = tcp / a.wmstr * a.gasc * a.tstr / a.hstr;
+  a.ekt[c] = a.rei * (a.beta2 * a.vtc[c] / tcp + a.tmu[c] / a.sigh);
+}
+
+// E, W, P (J+1), Q (J-1) of the 51 species, the soot scalars (STE..STS) and the enthalpy
+// (ZE..ZS) of one interior cell with ISKIP < 10 (spgsolv.f:1113-1290): hybrid upwinding of
+// the face mass flux against the face conductance (spgsolv.f:233-349, formed here from the
+// two cells beside the face) and the axisymmetric source on the J faces.  The soot W
+// coefficient keeps the reference's comparison with the enthalpy's upwinded value
+// (spgsolv.f:1208).  One thread per cell, J fastest across lanes.
+__global__ void __launch_bounds__(256) k_asm_coef(AsmArgs a) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t nj = a.lj - 2;
+  if (t >= (int64_t)(a.li - 2) * nj) return;
+  const int32_t i = 1 + (int32_t)(t / nj), j = 1 + (int32_t)(t % nj);
+  const int64_t c = (int64_t)i * a.lj + j, n = a.n;
+  if (a.iskip[c] >= 10) return;
+  const int64_t ce = c + a.lj, cw = c - a.lj, cn = c + 1, cs = c - 1;
+  const double dyn = a.yys[j + 1], dys = a.yys[j];
+  const double dtbrx = a.dt / a.xxc[i], dtbry = a.dt / a.yyc[j];
+  const double rhoue = 0.5 * (a.rho[ce] + a.rho[c]) * a.u[ce];
+  const double rhouw = 0.5 * (a.rho[c] + a.rho[cw]) * a.u[c];
+  const double rhovn = 0.5 * (a.rho[cn] + a.rho[c]) * a.v[cn];
+  const double rhovs = 0.5 * (a.rho[c] + a.rho[cs]) * a.v[c];
+  const double rb = a.rei * a.beta3;
+  const double emc = rb * a.rho[c], eme = rb * a.rho[ce], emw = rb * a.rho[cw], emn = rb * a.rho[cn],
+               ems = rb * a.rho[cs];
+  const double yg = a.ygeo[c];
+  const double symz = a.sym * a.dt * 0.5 * a.ekt[c] / yg;
+  const double sym1 = a.sym * a.dt * 0.5 * emc / yg;
+  const double sym2 = a.sym * a.dt * 0.5 * a.rei * a.tmu[c] / a.sigsp / yg;
+  const double turw_e = a.rei * (a.tmu[ce] + a.tmu[c]) / a.sigsp;
+  const double turw_w = a.rei * (a.tmu[c] + a.tmu[cw]) / a.sigsp;
+  const double turs_n = a.rei * (a.tmu[cn] + a.tmu[c]) / a.sigsp;
+  const double turs_s = a.rei * (a.tmu[c] + a.tmu[cs]) / a.sigsp;
+  const double xe = a.xxs[i + 1], xw = a.xxs[i];
+  const double are = fabs(rhoue), arw = fabs(rhouw), arn = fabs(rhovn), ars = fabs(rhovs);
+  // one scalar's four coefficients from its face conductances and symmetry term
+  auto put = [&](double* o, int64_t stride, double de, double dw, double dn, double ds, double symc,
+                 double wconv_ref, bool quirk) {
+    double conv = are;
+    if (de >= conv) conv = de;
+    o[0] = dtbrx * 0.5 * (rhoue - conv);
+    conv = arw;
+    if (dw >= (quirk ? wconv_ref : conv)) conv = dw;
+    o[stride] = dtbrx * 0.5 * (-rhouw - conv);
+    conv = arn;
+    if (dn >= conv) conv = dn;
+    o[2 * stride] = dtbry * 0.5 * (rhovn - conv) - symc / dyn;
+    conv = ars;
+    if (ds >= conv) conv = ds;
+    o[3 * stride] = dtbry * 0.5 * (-rhovs - conv) + symc / dys;
+  };
+  // enthalpy (its W upwinded value is the soot W quirk's reference)
+  const double ekc = a.ekt[c];
+  const double zdw = (ekc + a.ekt[cw]) / xw;
+  double zconv_w = arw;
+  if (zdw >= zconv_w) zconv_w = zdw;
+  put(a.zc + c, n, (a.ekt[ce] + ekc) / xe, zdw, (a.ekt[cn] + ekc) / dyn, (ekc + a.ekt[cs]) / dys, symz, 0.0, false);
+  // soot scalars (STDF) and the species (VDF)
+  for (int s = 51; s >= 0; --s) {
+    const double* df = s == 51 ? a.stdf : a.vdf + (int64_t)s * n;
+    const double dc = df[c];
+    const double de = (eme * df[ce] + emc * dc + turw_e) / xe;
+    const double dw = (emc * dc + emw * df[cw] + turw_w) / xw;
+    const double dn = (emn * df[cn] + emc * dc + turs_n) / dyn;
+    const double ds = (emc * dc + ems * df[cs] + turs_s) / dys;
+    double* o = s == 51 ? a.stc + c : a.coef + (int64_t)(4 * s) * n + c;
+    put(o, n, de, dw, dn, ds, sym1 * dc + sym2, zconv_w, s == 51);
+  }
+}
+
 extern "C" {
 
 int uc_abi_version(void) { return UC_ABI_VERSION; }
@@ -1235,6 +1338,39 @@ int uc_destroy(uc_context* ctx) {
 
 int uc_step(uc_context* ctx, const uc_field* f, const uc_controls* ctl, void* stream, uc_stats* st) {
   return step_impl(ctx, f, ctl, (hipStream_t)stream, st);
+}
+
+int uc_assemble(uc_context* ctx, const uc_field* f, const uc_controls* ctl, const uc_transport* tr, double* coef,
+                double* stc, double* zc, double* ekt, void* stream) {
+  if (!ctx || !f || !ctl || !tr) return fail(UC_E_ARG, "uc_assemble: null argument");
+  if (f->li < 3 || f->lj < 3) return fail(UC_E_ARG, "uc_assemble: grid must be at least 3 x 3");
+  const int64_t n = (int64_t)f->li * f->lj;
+  if (n > ctx->max_cells) return fail(UC_E_ARG, "uc_assemble: the grid does not fit the context");
+  if (!f->rho || !f->y || !f->tk || !f->iskip)
+    return fail(UC_E_ARG, "uc_assemble: null field array (rho, y, tk and iskip are read)");
+  if (!tr->vtc || !tr->tmu || !tr->vdf || !tr->stdf || !tr->u || !tr->v || !tr->ygeo || !tr->xxc || !tr->xxs ||
+      !tr->yyc || !tr->yys)
+    return fail(UC_E_ARG, "uc_assemble: null transport array");
+  if (!coef || !stc || !zc || !ekt) return fail(UC_E_ARG, "uc_assemble: null output array");
+  if (!(ctl->dt > 0.0) || !(tr->sigsp > 0.0) || !(tr->sigh > 0.0))
+    return fail(UC_E_ARG, "uc_assemble: dt, sigsp and sigh must be positive");
+  HIPCHK(hipSetDevice(ctx->device));
+  AsmArgs a{};
+  a.rho = f->rho; a.y = f->y; a.tk = f->tk; a.iskip = f->iskip;
+  a.vtc = tr->vtc; a.tmu = tr->tmu; a.vdf = tr->vdf; a.stdf = tr->stdf; a.u = tr->u; a.v = tr->v;
+  a.ygeo = tr->ygeo; a.xxc = tr->xxc; a.xxs = tr->xxs; a.yyc = tr->yyc; a.yys = tr->yys;
+  a.coef = coef; a.stc = stc; a.zc = zc; a.ekt = ekt;
+  a.n = n; a.li = f->li; a.lj = f->lj;
+  a.dt = ctl->dt; a.sigsp = tr->sigsp; a.sigh = tr->sigh; a.sym = (double)tr->isym;
+  const uc_scales& S = ctx->sc;
+  a.rei = S.rei; a.beta2 = S.beta2; a.beta3 = S.beta3; a.tstr = S.tstr; a.gasc = S.gasc; a.wmstr = S.wmstr;
+  a.hstr = S.hstr;
+  hipStream_t s = (hipStream_t)stream;
+  k_asm_ekt<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ctx->dp, a);
+  const int64_t ni = (int64_t)(f->li - 2) * (f->lj - 2);
+  k_asm_coef<<<(unsigned)((ni + 255) / 256), 256, 0, s>>>(a);
+  HIPCHK(hipGetLastError());
+  return UC_OK;
 }
 
 int uc_step_coupled(uc_context* ctx, const uc_field* f, const uc_controls* ctl, const uc_coupling* cp, void* stream,

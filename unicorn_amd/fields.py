@@ -302,3 +302,67 @@ def synthetic_coupling(f: ChemField, seed: int = 0, strength: float = 0.1, flux:
     cp.tmu = tmu * s
     cp.ekt = 0.1 * ekt * s
     return cp
+
+
+@dataclass
+class Transport:
+    """SPGSOLV's inputs to its coefficient assembly beyond the chemistry state
+    (spgsolv.f:98-349), in the field layout: thermal conductivity VTC, turbulent
+    viscosity TMU, species diffusivities VDF (51 used), soot diffusivity STDF, the face
+    velocities U (U(J,I) on the face between I-1 and I) and V (between J-1 and J), and
+    the geometry (radius Y, XXC/XXS, YYC/YYS, ISYM); SIGSP and SIGH as SPGSOLV receives
+    them (main:346-347, both 0.9)."""
+    vtc: np.ndarray      # (n,)
+    tmu: np.ndarray      # (n,)
+    vdf: np.ndarray      # (52, n)
+    stdf: np.ndarray     # (n,)
+    u: np.ndarray        # (n,)
+    v: np.ndarray        # (n,)
+    ygeo: np.ndarray     # (n,)
+    xxc: np.ndarray      # (li,)
+    xxs: np.ndarray      # (li,)
+    yyc: np.ndarray      # (lj,)
+    yys: np.ndarray      # (lj,)
+    sigsp: float = float(np.float32(0.9))
+    sigh: float = float(np.float32(0.9))
+    isym: int = 1
+
+    def coupling(self, coef, stc, zc, ekt, w=None) -> Coupling:
+        """The coupling inputs of the chemistry step from assembled coefficients (numpy
+        arrays): SOLVESP's and SOLVEH's view of what SPGSOLV has set."""
+        n = self.tmu.size
+        return Coupling(np.asarray(coef), np.asarray(stc), np.asarray(zc), np.asarray(ekt), self.tmu, self.vdf,
+                        self.ygeo, self.xxc, self.xxs, self.yyc, self.yys, self.u, self.v,
+                        np.zeros(n) if w is None else w, sigsp=self.sigsp, isym=self.isym)
+
+
+def synthetic_transport(f: ChemField, seed: int = 0, strength: float = 0.02, velocity: float = 0.3,
+                        scales: Scales | None = None) -> Transport:
+    """Seeded transport inputs for a field: uniform spacing on the unit square, radius
+    Y = J*dy, ISYM = 1; VDF uniform in [0.5, 2], TMU in [0, 1] and STDF in [0, 0.1]
+    times scales chosen so that a face's diffusive coefficient DT/DX * conductance is
+    about `strength` times RHO, VTC so that EKT is of the same order; velocities uniform
+    in [-velocity, velocity]."""
+    sc = scales or Scales()
+    rng = np.random.Generator(np.random.PCG64(seed))
+    li, lj, n = f.li, f.lj, f.n
+    dx, dy = 1.0 / (li - 1), 1.0 / (lj - 1)
+    rho = float(np.median(f.rho))
+    dscale = strength * min(dx, dy) ** 2 / (f.dt * sc.rei * sc.beta3)
+    vdf = rng.uniform(0.5, 2.0, (LSP, n)) * dscale
+    tmu = rng.uniform(0.0, 1.0, n) * dscale * sc.beta3 * rho
+    stdf = rng.uniform(0.0, 0.1, n) * dscale
+    # EKT = REI (BETA2 VTC / CP + TMU / SIGH) of about the species' conductance scale
+    pol = nasa7()
+    T = f.tk * sc.tstr
+    hi = T > 1000.0
+    cp = np.zeros(n)
+    for k in range(LSP):
+        a = np.where(hi[None, :], pol[k, 7:12, None], pol[k, 0:5, None])
+        cp += (a[0] + a[1] * T + a[2] * T ** 2 + a[3] * T ** 3 + a[4] * T ** 4) / sc.wm[k] * f.y[k]
+    cp = cp / sc.wmstr * sc.gasc * sc.tstr / sc.hstr
+    ekt = rng.uniform(0.5, 1.5, n) * dscale * sc.beta3 * rho
+    vtc = np.maximum(ekt - tmu / float(np.float32(0.9)), 0.0) * cp / sc.beta2
+    return Transport(vtc, tmu, vdf, stdf, rng.uniform(-velocity, velocity, n), rng.uniform(-velocity, velocity, n),
+                     np.tile(np.arange(lj) * dy, li), np.full(li, dx), np.full(li, dx), np.full(lj, dy),
+                     np.full(lj, dy))

@@ -92,6 +92,7 @@ class Scales:
         d["wmstr"] = wmref
         d["rei"] = amref / rref / vref / d["alstr"]
         d["beta1"] = GASC * d["rstr"] * d["tstr"] / d["pstr"] / d["wmstr"]
+        d["beta2"] = d["tstr"] * d["acstr"] / d["amstr"] / d["vstr"] / d["vstr"]
         d["beta3"] = d["rstr"] * d["dstr"] / d["amstr"]
         d["pback"] = pref / d["pstr"]
         d["hconv"] = GASC / d["wmstr"] / d["hstr"]
