@@ -391,7 +391,7 @@ def main():
                                          "out every step (first steps, so more sweeps than the 20-step average)"}
     if not args.no_coupled:
         c = coupled_bench(args, f0)
-        out["coupled"] = {k: c[k] for k in ("value", "ms_per_step", "roofline", "wavefront", "sweeps")}
+        out["coupled"] = {k: c[k] for k in ("value", "ms_per_step", "roofline", "wavefront", "sweeps", "assemble")}
         if "cpu_baseline" in c:
             out["coupled"]["cpu_baseline"] = c["cpu_baseline"]
             out["coupled"]["gpu_over_cpu"] = c["gpu_over_cpu"]

@@ -1128,8 +1128,8 @@ __global__ void __launch_bounds__(256) k_asm_ekt(const UcParams* __restrict__ P,
   const double tkd2 = tkd * tkd, tkd3 = tkd * tkd2, tkd4 = tkd * tkd3;
   const int ip = tkd > 1000.0 ? 7 : 0;  // /CB08/ TPOL1
   double tcp = 0.0;
-#pragma unroll 4
-  for (int k = 0; k < NSP; ++k) {
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) {  // unrolled: the 52 Y loads are in flight together
     const double* p = &P->nasa[k][ip];
     const double cpk = (p[0] + p[1] * tkd + p[2] * tkd2 + p[3] * tkd3 + p[4] * tkd4) / P->wm[k];
     tcp = tcp + cpk * a.y[(int64_t)k * a.n + c];
@@ -1194,7 +1194,8 @@ __global__ void __launch_bounds__(256) k_asm_coef(AsmArgs a) {
   double zconv_w = arw;
   if (zdw >= zconv_w) zconv_w = zdw;
   put(a.zc + c, n, (a.ekt[ce] + ekc) / xe, zdw, (a.ekt[cn] + ekc) / dyn, (ekc + a.ekt[cs]) / dys, symz, 0.0, false);
-  // soot scalars (STDF) and the species (VDF)
+  // soot scalars (STDF) and the species (VDF); unrolled so several scalars' loads overlap
+#pragma unroll 4
   for (int s = 51; s >= 0; --s) {
     const double* df = s == 51 ? a.stdf : a.vdf + (int64_t)s * n;
     const double dc = df[c];
