@@ -232,6 +232,15 @@ int uc_get_tuning(const uc_context *ctx, uc_tuning *tuning);
 void ckrates_(void);
 void solvesp_(int32_t *isor1, double *relxsp, double *tolrsp, double *spnorm);
 void solveh_(int32_t *isor2, double *relxh, double *tolrh, double *sigsp, double *hnorm);
+/* Replaces SUBROUTINE SPGSOLV(ISOR1,RELXSP,TOLRSP,RESDC,ISOR2,RELXH,TOLRH,RESDH,SIGH,SIGSP)
+ * (src/lib/spgsolv.f:6) as a whole, for a Fortran build that drops spgsolv.f instead of its
+ * three INCLUDE lines: the coefficient assembly on the device (uc_assemble) from the
+ * transport COMMONs, the three routines above with those coefficients, and the epilogue
+ * (RESDC, RESDH, FA = SA, HT = FPZ, STMF/STND = SST).  The scratch COMMONs SPGSOLV fills on
+ * the way (/DUMMY/, /DUM0/, /DUMX/, /DUMS/ STE..STS, /DUMY/ EMU and EKT) are not written.
+ * Needs ICHEM > 0 and ITHRM > 0 (stops with a message otherwise). */
+void spgsolv_(int32_t *isor1, double *relxsp, double *tolrsp, double *resdc, int32_t *isor2, double *relxh,
+              double *tolrh, double *resdh, double *sigh, double *sigsp);
 
 #ifdef __cplusplus
 }

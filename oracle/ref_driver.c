@@ -190,13 +190,18 @@ static double now(void) {
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
-/* SPGSOLV's coefficient assembly alone (ICHEM = ITHRM = 0: spgsolv.f:98-1290, then its
- * epilogue, which with SA = FA changes nothing read here).  Input 'UCA1': nj, ni, isym
- * (int64); dt, sigsp, sigh, ref_values[6]; rho[n], y[52][n], tk[n], iskip[n] (int32),
+static void clip_closure(void);
+
+/* SPGSOLV's coefficient assembly alone (mode "asm": ICHEM = ITHRM = 0, spgsolv.f:98-1290,
+ * then its epilogue, which with SA = FA changes nothing read here), or the whole of SPGSOLV
+ * after main:821-828 (mode "spg": ICHEM = ITHRM = 1; ISOR 100, RELX 0.99, TOL 1e-8).
+ * Input 'UCA1': nj, ni, isym (int64); dt, sigsp, sigh, ref_values[6]; rho[n], y[52][n],
+ * tk[n], ht[n], stmf[n], stnd[n], iskip[n] (int32),
  * vtc[n], tmu[n], vdf[52][n], stdf[n], u[n], v[n], ygeo[n], xxc[ni], xxs[ni], yyc[nj],
- * yys[nj] (field layout, j fastest).  Output: ekt[n], emu[n], coef[204][n] (/DUMMY/ then
- * /DUM0/), stc[4][n] (STE..STS), zc[4][n] (ZE..ZS). */
-static int run_assemble(const char *inp, const char *outp) {
+ * yys[nj] (field layout, j fastest).  Output of "asm": ekt[n], emu[n], coef[204][n]
+ * (/DUMMY/ then /DUM0/), stc[4][n] (STE..STS), zc[4][n] (ZE..ZS); of "spg": isor1, isor2
+ * (int64), resdc, resdh, then y[52][n], tk, ht, stmf, stnd, qdot. */
+static int run_assemble(const char *inp, const char *outp, int full) {
     if (!spgsolv_) die("this driver has no SPGSOLV (build ref_driver_asm)");
     FILE *f = fopen(inp, "rb");
     if (!f) die("cannot open input");
@@ -208,12 +213,14 @@ static int run_assemble(const char *inp, const char *outp) {
     double sc[9];
     rd(sc, sizeof sc, f);
     const size_t n = (size_t)nj * ni;
+    double *ht = malloc(8 * n), *stm = malloc(8 * n), *stn = malloc(8 * n);
     double *rho = malloc(8 * n), *y = malloc(8 * n * LSP), *tk = malloc(8 * n), *vtc = malloc(8 * n),
            *tmu = malloc(8 * n), *vdf = malloc(8 * n * LSP), *stdf = malloc(8 * n), *u = malloc(8 * n),
            *v = malloc(8 * n), *yg = malloc(8 * n), *xxc = malloc(8 * ni), *xxs = malloc(8 * ni),
            *yyc = malloc(8 * nj), *yys = malloc(8 * nj);
     int32_t *isk = malloc(4 * n);
-    rd(rho, 8 * n, f); rd(y, 8 * n * LSP, f); rd(tk, 8 * n, f); rd(isk, 4 * n, f);
+    rd(rho, 8 * n, f); rd(y, 8 * n * LSP, f); rd(tk, 8 * n, f);
+    rd(ht, 8 * n, f); rd(stm, 8 * n, f); rd(stn, 8 * n, f); rd(isk, 4 * n, f);
     rd(vtc, 8 * n, f); rd(tmu, 8 * n, f); rd(vdf, 8 * n * LSP, f); rd(stdf, 8 * n, f);
     rd(u, 8 * n, f); rd(v, 8 * n, f); rd(yg, 8 * n, f);
     rd(xxc, 8 * ni, f); rd(xxs, 8 * ni, f); rd(yyc, 8 * nj, f); rd(yys, 8 * nj, f);
@@ -239,6 +246,7 @@ static int run_assemble(const char *inp, const char *outp) {
             const int c = fidx(j, i);
             cb01_.iskip[c] = isk[q];
             cb04_.rho[c] = rho[q]; cb04_.tk[c] = tk[q]; cb04_.u[c] = u[q]; cb04_.v[c] = v[q];
+            cb04_.ht[c] = ht[q]; soot_.stmf[c] = stm[q]; soot_.stnd[c] = stn[q];
             for (int k = 0; k < LSP; ++k) {
                 cb04_.fsp[k][c] = y[(size_t)k * n + q];
                 cb07_.vdf[k][c] = vdf[(size_t)k * n + q];
@@ -248,34 +256,49 @@ static int run_assemble(const char *inp, const char *outp) {
         }
     cb03_.dt = sc[0];
     cb01_.isym = (int32_t)hdr[3];
-    cb01_.ichem = 0;
-    cb01_.ithrm = 0;
-    int32_t isor1 = 1, isor2 = 1;
-    double relxsp = 1.0, tolrsp = 1.0, resdc = 0.0, relxh = 1.0, tolrh = 1.0, resdh = 0.0;
+    cb01_.ichem = full ? 1 : 0;
+    cb01_.ithrm = full ? 1 : 0;
+    int32_t isor1 = full ? 100 : 1, isor2 = full ? 100 : 1;
+    double relxsp = full ? 0.99 : 1.0, tolrsp = full ? 1e-8 : 1.0, resdc = 0.0;
+    double relxh = full ? 0.99 : 1.0, tolrh = full ? 1e-8 : 1.0, resdh = 0.0;
     double sigsp = sc[1], sigh = sc[2];
+    if (full) clip_closure();
     spgsolv_(&isor1, &relxsp, &tolrsp, &resdc, &isor2, &relxh, &tolrh, &resdh, &sigh, &sigsp);
     FILE *fo = fopen(outp, "wb");
     if (!fo) die("cannot open output");
     double *buf = malloc(8 * n);
+    if (full) {
+        int64_t cnt[2] = {isor1, isor2};
+        double res[2] = {resdc, resdh};
+        wr(cnt, sizeof cnt, fo);
+        wr(res, sizeof res, fo);
+    }
 #define DUMPA(expr)                                                                  \
     do {                                                                             \
         for (int i = 0; i < ni; ++i)                                                 \
             for (int j = 0; j < nj; ++j) { int c = fidx(j, i); buf[(size_t)i * nj + j] = (expr); } \
         wr(buf, 8 * n, fo);                                                          \
     } while (0)
-    DUMPA(dumy_.ekt[c]);
-    DUMPA(dumy_.emu[c]);
-    for (int g = 0; g < 204; ++g) DUMPA(g < 55 ? dummy_.a[g][c] : dum0_.a[g - 55][c]);
-    DUMPA(dums_.ste[c]); DUMPA(dums_.stw[c]); DUMPA(dums_.stn[c]); DUMPA(dums_.sts[c]);
-    DUMPA(dumx_.ze[c]); DUMPA(dumx_.zw[c]); DUMPA(dumx_.zn[c]); DUMPA(dumx_.zs[c]);
+    if (full) {
+        for (int k = 0; k < LSP; ++k) DUMPA(cb04_.fsp[k][c]);
+        DUMPA(cb04_.tk[c]); DUMPA(cb04_.ht[c]); DUMPA(soot_.stmf[c]); DUMPA(soot_.stnd[c]);
+        DUMPA(hrls_.qdot[c]);
+    } else {
+        DUMPA(dumy_.ekt[c]);
+        DUMPA(dumy_.emu[c]);
+        for (int g = 0; g < 204; ++g) DUMPA(g < 55 ? dummy_.a[g][c] : dum0_.a[g - 55][c]);
+        DUMPA(dums_.ste[c]); DUMPA(dums_.stw[c]); DUMPA(dums_.stn[c]); DUMPA(dums_.sts[c]);
+        DUMPA(dumx_.ze[c]); DUMPA(dumx_.zw[c]); DUMPA(dumx_.zn[c]); DUMPA(dumx_.zs[c]);
+    }
 #undef DUMPA
     fclose(fo);
     return 0;
 }
 
 int main(int argc, char **argv) {
-    if (argc < 3) die("usage: ref_driver mech <out> | run <in> <out> | asm <in> <out>");
-    if (!strcmp(argv[1], "asm") && argc >= 4) return run_assemble(argv[2], argv[3]);
+    if (argc < 3) die("usage: ref_driver mech <out> | run <in> <out> | asm <in> <out> | spg <in> <out>");
+    if (!strcmp(argv[1], "asm") && argc >= 4) return run_assemble(argv[2], argv[3], 0);
+    if (!strcmp(argv[1], "spg") && argc >= 4) return run_assemble(argv[2], argv[3], 1);
     if (!strcmp(argv[1], "mech")) {
         double rv[6] = {10.0, 294.0, 1.0133e+05, 0.0, 0.100, 0.233};
         setup(rv);

@@ -37,6 +37,45 @@ def available(driver: str = "ref") -> bool:
 ASM_OUT = ("ekt", "emu", "coef", "stc", "zc")
 
 
+def spgsolv(field, tr, driver="asm", ref_values=(10.0, 294.0, 1.0133e+05, 0.0, 0.100, 0.233)) -> dict:
+    """The whole of SPGSOLV after main:821-828 (ICHEM = ITHRM = 1; ISOR 100, RELX 0.99,
+    TOL 1e-8): driver "asm" runs the flang-built spgsolv.f, "shim" the drop-in's spgsolv_
+    (libunicorn_fshim.so).  Returns isor1, isor2, resdc, resdh and the new y, tk, ht,
+    stmf, stnd, qdot (field layout)."""
+    exe = SHIM_DRIVER if driver == "shim" else ASM_DRIVER
+    if not exe.exists():
+        raise FileNotFoundError(f"{exe} not built (make -C oracle asm shim)")
+    n = field.n
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        _write_asm_input(fin, field, tr, ref_values)
+        subprocess.run([str(exe), "spg", fin, fout], check=True, capture_output=True, text=True)
+        raw = np.fromfile(fout, dtype=np.uint8)
+    cnt = raw[:16].view(np.int64)
+    res = raw[16:32].view(np.float64)
+    a = raw[32:].view(np.float64)
+    out = {"isor1": int(cnt[0]), "isor2": int(cnt[1]), "resdc": float(res[0]), "resdh": float(res[1]),
+           "y": a[:52 * n].reshape(52, n)}
+    o = 52 * n
+    for k in ("tk", "ht", "stmf", "stnd", "qdot"):
+        out[k] = a[o:o + n]
+        o += n
+    assert o == a.size
+    return out
+
+
+def _write_asm_input(fin, field, tr, ref_values):
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64).tobytes()
+    with open(fin, "wb") as fh:
+        fh.write(np.array([0x31414355, field.lj, field.li, tr.isym], dtype=np.int64).tobytes())
+        fh.write(np.array([field.dt, tr.sigsp, tr.sigh, *ref_values], dtype=np.float64).tobytes())
+        for a in (field.rho, field.y, field.tk, field.ht, field.stmf, field.stnd):
+            fh.write(f64(a))
+        fh.write(np.ascontiguousarray(field.iskip, dtype=np.int32).tobytes())
+        for a in (tr.vtc, tr.tmu, tr.vdf, tr.stdf, tr.u, tr.v, tr.ygeo, tr.xxc, tr.xxs, tr.yyc, tr.yys):
+            fh.write(f64(a))
+
+
 def assemble(field, tr, ref_values=(10.0, 294.0, 1.0133e+05, 0.0, 0.100, 0.233)) -> dict:
     """SPGSOLV's coefficient assembly by the flang-built spgsolv.f (ICHEM = ITHRM = 0) on
     a field and unicorn_amd.fields.Transport inputs: ekt, emu (n,), coef (204, n),
@@ -45,17 +84,9 @@ def assemble(field, tr, ref_values=(10.0, 294.0, 1.0133e+05, 0.0, 0.100, 0.233))
     if not ASM_DRIVER.exists():
         raise FileNotFoundError(f"{ASM_DRIVER} not built (make -C oracle asm)")
     n = field.n
-    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64).tobytes()
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
-        with open(fin, "wb") as fh:
-            fh.write(np.array([0x31414355, field.lj, field.li, tr.isym], dtype=np.int64).tobytes())
-            fh.write(np.array([field.dt, tr.sigsp, tr.sigh, *ref_values], dtype=np.float64).tobytes())
-            for a in (field.rho, field.y, field.tk):
-                fh.write(f64(a))
-            fh.write(np.ascontiguousarray(field.iskip, dtype=np.int32).tobytes())
-            for a in (tr.vtc, tr.tmu, tr.vdf, tr.stdf, tr.u, tr.v, tr.ygeo, tr.xxc, tr.xxs, tr.yyc, tr.yys):
-                fh.write(f64(a))
+        _write_asm_input(fin, field, tr, ref_values)
         subprocess.run([str(ASM_DRIVER), "asm", fin, fout], check=True, capture_output=True, text=True)
         raw = np.fromfile(fout, dtype=np.float64)
     sizes = {"ekt": 1, "emu": 1, "coef": 204, "stc": 4, "zc": 4}

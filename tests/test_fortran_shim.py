@@ -120,3 +120,31 @@ def test_shim_matches_reference_fixtures(name):
         assert_state_close(got, e, inter, f"shim {name}/{ns}")
         dump = z["dump_cells"]
         assert_rates_close(r.qdot, r.qdsp[:, dump], e["qdot"], e["qdsp_dump"], inter, f"shim {name}/{ns}")
+
+
+@pytest.mark.gpu
+def test_shim_spgsolv_matches_reference():
+    """The drop-in's spgsolv_ (coefficient assembly on the device, then the coupled
+    chemistry, then SPGSOLV's epilogue) against the flang-built spgsolv.f on the same
+    COMMON blocks (tests/golden/assemble.npz's field and transport inputs): sweep counts
+    exact, RESDC/RESDH and the state to the parity bar.  Both normalise the global stops
+    by the Fortran build's 711 x 131 grid (spgsolv.f:100-101)."""
+    from golden_io import load_transport
+    if not (refrun.available("shim") and refrun.available("asm")):
+        pytest.skip("reference SPGSOLV and shim drivers not built")
+    f0, z = load("assemble")
+    tr = load_transport(z)
+    ref = refrun.spgsolv(f0.copy(), tr, driver="asm")
+    print("shim spgsolv_", ref["isor1"], ref["isor2"], flush=True)
+    got = refrun.spgsolv(f0.copy(), tr, driver="shim")
+    inter = f0.interior()
+    print("shim", got["isor1"], got["isor2"], "max |dY|", np.abs(got["y"][:, inter] - ref["y"][:, inter]).max(),
+          "max |dT|/T", np.abs(got["tk"][inter] - ref["tk"][inter]).max() / np.abs(ref["tk"][inter]).max(),
+          "max |dHT|", np.abs(got["ht"][inter] - ref["ht"][inter]).max(), flush=True)
+    assert (got["isor1"], got["isor2"]) == (ref["isor1"], ref["isor2"])
+    assert np.isclose(got["resdc"], ref["resdc"], rtol=1e-9, atol=0)
+    assert np.isclose(got["resdh"], ref["resdh"], rtol=1e-9, atol=0)
+    g = ChemField(f0.li, f0.lj, f0.rho, got["y"], got["tk"], got["ht"], got["stmf"], got["stnd"], f0.iskip, f0.dt,
+                  f0.fuel)
+    assert_state_close(g, {k: ref[k] for k in ("y", "tk", "ht", "stmf", "stnd")} | {"rho": f0.rho}, inter,
+                       "spgsolv_")

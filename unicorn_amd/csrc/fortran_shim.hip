@@ -61,7 +61,7 @@ extern struct { double alstr, vstr, rstr, pstr, tstr, tmstr, amstr, acstr, dstr,
 extern struct { double hfo[shim::LSP]; } hform_;
 extern struct { double qdot[shim::LE]; } hrls_;
 extern struct { double qdsp[shim::LSP][shim::LE]; } dumd_;
-extern struct { double rsoot, stmf[shim::LE], stnd[shim::LE]; } soot_;
+extern struct { double rsoot, stmf[shim::LE], stnd[shim::LE], stdf[shim::LE]; } soot_;
 extern struct { double a[55][shim::LE]; } dummy_;
 extern struct { double a[149][shim::LE]; } dum0_;
 extern struct { double z[4][shim::LE]; } dumx_;
@@ -95,6 +95,7 @@ struct Shim {
   double *coef = nullptr, *stc = nullptr, *zc = nullptr, *emu = nullptr, *ekt = nullptr, *tmu = nullptr,
          *vdf = nullptr, *ygeo = nullptr, *xxc = nullptr, *xxs = nullptr, *yyc = nullptr, *yys = nullptr,
          *u = nullptr, *v = nullptr, *w = nullptr;
+  double *vtc = nullptr, *stdf = nullptr;  // spgsolv_'s further assembly inputs
   uc_coupling cp{};
   bool coupled = false;
   // QDOT/QDSP on the device mirror the COMMONs once uploaded: only SOLVEH writes them
@@ -219,10 +220,18 @@ void d2h(void* dst, const void* src, size_t b, const char* where) {
 
 extern "C" {
 
-// SUBROUTINE CKRATES (src/lib/ckrates.f:1)
-void ckrates_(void) {
-  const Present pr = scan_coupling();  // host COMMONs only
-  ensure_context();
+}  // extern "C"
+
+namespace {
+
+double* dalloc(double*& p, size_t bytes, const char* where) {
+  if (!p) hipcheck(hipMalloc(&p, bytes), where);
+  return p;
+}
+
+// the state CKRATES reads, and SPGSOLV's prologue copies (spgsolv.f:169-231) made on the
+// device: SA = FA, SST1 = STMF, SST2 = STND and FPZ = HT at every cell
+void upload_state() {
   const size_t b1 = sizeof(double) * shim::LE, b52 = b1 * shim::LSP;
   h2d(g.rho, cb04_.rho, b1, "ckrates_");
   h2d(g.y, cb04_.fsp, b52, "ckrates_");
@@ -246,11 +255,32 @@ void ckrates_(void) {
     h2d(g.qdsp, dumd_.qdsp, b52, "ckrates_");
     g.q_resident = true;
   }
+}
+
+// CKRATES on the device after the coupling inputs are set: the step's cell list, the
+// rate coefficients
+void begin_rates(const char* where) {
+  hipcheck(hipStreamSynchronize(g.s), where);
+  g.ctx->lay.ngrids = 0;  // ISKIP may change between steps: rebuild the cell list
+  // SOLVESP/SOLVEH supply the controls and norms; these are placeholders until then
+  const uc_controls ctl{cb03_.dt, 0.99, 1e-8, 0.99, 1e-8, 100, 100, UC_MODE_GLOBAL, 0};
+  check(ph_begin(g.ctx, &g.f, &ctl, 1, g.s, false, nullptr, nullptr, g.coupled ? &g.cp : nullptr), where);
+  check(ph_rates(g.ctx), where);
+  g.stage = 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+// SUBROUTINE CKRATES (src/lib/ckrates.f:1)
+void ckrates_(void) {
+  const Present pr = scan_coupling();  // host COMMONs only
+  ensure_context();
+  upload_state();
+  const size_t b1 = sizeof(double) * shim::LE, b52 = b1 * shim::LSP;
   // the coupling inputs this step has
-  auto dev = [&](double*& p, size_t bytes) {
-    if (!p) hipcheck(hipMalloc(&p, bytes), "ckrates_: hipMalloc");
-    return p;
-  };
+  auto dev = [&](double*& p, size_t bytes) { return dalloc(p, bytes, "ckrates_: hipMalloc"); };
   g.cp = uc_coupling{};
   if (pr.coef) {
     dev(g.coef, 204 * b1);
@@ -282,13 +312,7 @@ void ckrates_(void) {
   g.cp.sigsp = 0.9;  // SOLVEH's argument, set there
   g.cp.isym = cb01_.isym;
   g.coupled = pr.coef || pr.st || pr.z || pr.flux || pr.vel;
-  hipcheck(hipStreamSynchronize(g.s), "ckrates_");
-  g.ctx->lay.ngrids = 0;  // ISKIP may change between steps: rebuild the cell list
-  // SOLVESP/SOLVEH supply the controls and norms; these are placeholders until then
-  const uc_controls ctl{cb03_.dt, 0.99, 1e-8, 0.99, 1e-8, 100, 100, UC_MODE_GLOBAL, 0};
-  check(ph_begin(g.ctx, &g.f, &ctl, 1, g.s, false, nullptr, nullptr, g.coupled ? &g.cp : nullptr), "ckrates_");
-  check(ph_rates(g.ctx), "ckrates_");
-  g.stage = 1;
+  begin_rates("ckrates_");
 }
 
 // SUBROUTINE SOLVESP(ISOR1,RELXSP,TOLRSP,SPNORM) (src/lib/solvesp.f:1)
@@ -369,6 +393,72 @@ void solveh_(int32_t* isor2, double* relxh, double* tolrh, double* sigsp, double
   d2h(dumd_.qdsp, g.qdsp, b1 * shim::LSP, "solveh_");
   hipcheck(hipStreamSynchronize(g.s), "solveh_");
   g.stage = 0;
+}
+
+// SUBROUTINE SPGSOLV(ISOR1,RELXSP,TOLRSP,RESDC,ISOR2,RELXH,TOLRH,RESDH,SIGH,SIGSP)
+// (src/lib/spgsolv.f:6), for a Fortran build that links this library in place of the whole
+// of spgsolv.f: the coefficient assembly (spgsolv.f:98-1290) runs on the device from the
+// transport COMMONs (/CB07/ VTC, TMU, VDF; /SOOT/ STDF; /CB04/ U, V; /CB03/ geometry), then
+// CKRATES, SOLVESP and SOLVEH as above with those coefficients, then the epilogue
+// (spgsolv.f:1299-1416) on the host COMMONs.  The scratch COMMONs SPGSOLV fills on the way
+// (/DUMMY/, /DUM0/, /DUMX/, /DUMS/ STE..STS, /DUMY/ EMU and EKT) are not written: the main
+// program reuses them as work arrays.  Needs ICHEM > 0 and ITHRM > 0.
+void spgsolv_(int32_t* isor1, double* relxsp, double* tolrsp, double* resdc, int32_t* isor2, double* relxh,
+              double* tolrh, double* resdh, double* sigh, double* sigsp) {
+  if (cb01_.ichem <= 0 || cb01_.ithrm <= 0) stop("spgsolv_", "the device SPGSOLV needs ICHEM > 0 and ITHRM > 0");
+  ensure_context();
+  // SPGSOLV's FPZ = HT at every cell (spgsolv.f:176), which CKRATES's upload reads as HT
+  std::memcpy(dumy_.fpz, cb04_.ht, sizeof(double) * shim::LE);
+  upload_state();
+  const size_t b1 = sizeof(double) * shim::LE, b52 = b1 * shim::LSP;
+  const size_t bi = sizeof(double) * shim::LI, bj = sizeof(double) * shim::LJ;
+  const char* W = "spgsolv_";
+  h2d(dalloc(g.vtc, b1, W), cb07_.vtc, b1, W);
+  h2d(dalloc(g.tmu, b1, W), cb07_.tmu, b1, W);
+  h2d(dalloc(g.vdf, b52, W), cb07_.vdf, b52, W);
+  h2d(dalloc(g.stdf, b1, W), soot_.stdf, b1, W);
+  h2d(dalloc(g.u, b1, W), cb04_.u, b1, W);
+  h2d(dalloc(g.v, b1, W), cb04_.v, b1, W);
+  h2d(dalloc(g.w, b1, W), cb04_.w, b1, W);
+  h2d(dalloc(g.ygeo, b1, W), cb03_.y, b1, W);
+  h2d(dalloc(g.xxc, bi, W), cb03_.xxc, bi, W);
+  h2d(dalloc(g.xxs, bi, W), cb03_.xxs, bi, W);
+  h2d(dalloc(g.yyc, bj, W), cb03_.yyc, bj, W);
+  h2d(dalloc(g.yys, bj, W), cb03_.yys, bj, W);
+  // the assembly writes the interior cells with ISKIP < 10; elsewhere the coefficients are
+  // zero, as the reference's zero-initialised COMMONs hold them where nothing assembles
+  hipcheck(hipMemsetAsync(dalloc(g.coef, 204 * b1, W), 0, 204 * b1, g.s), W);
+  hipcheck(hipMemsetAsync(dalloc(g.stc, 4 * b1, W), 0, 4 * b1, g.s), W);
+  hipcheck(hipMemsetAsync(dalloc(g.zc, 4 * b1, W), 0, 4 * b1, g.s), W);
+  dalloc(g.ekt, b1, W);
+  const uc_transport tr{g.vtc, g.tmu, g.vdf, g.stdf, g.u, g.v, g.ygeo, g.xxc, g.xxs, g.yyc, g.yys, *sigsp, *sigh,
+                        cb01_.isym};
+  const uc_controls ctl{cb03_.dt, 0.99, 1e-8, 0.99, 1e-8, 100, 100, UC_MODE_GLOBAL, 0};
+  check(uc_assemble(g.ctx, &g.f, &ctl, &tr, g.coef, g.stc, g.zc, g.ekt, g.s), W);
+  // EMU null: REI*BETA3*RHO on the device, as spgsolv.f:175 sets it
+  g.cp = uc_coupling{g.coef, g.stc, g.zc, nullptr, g.ekt, g.tmu, g.vdf, g.ygeo, g.xxc, g.xxs, g.yyc, g.yys,
+                     g.u, g.v, g.w, *sigsp, cb01_.isym};
+  g.coupled = true;
+  begin_rates(W);
+  double spnorm = (double)shim::LE * (double)(shim::LSP - 1), hnorm = (double)shim::LE;  // spgsolv.f:100-101
+  solvesp_(isor1, relxsp, tolrsp, &spnorm);
+  solveh_(isor2, relxh, tolrh, sigsp, &hnorm);
+  // spgsolv.f:1299-1416: residuals over the cells that advanced, then the new state
+  double rh = 0.0, rc = 0.0;
+  for (int64_t i = 1; i < shim::LI - 1; ++i)
+    for (int64_t j = 1; j < shim::LJ - 1; ++j) {
+      const int64_t c = i * shim::LJ + j;
+      if (cb01_.iskip[c] >= 3) continue;
+      rh = rh + std::fabs(dumy_.fpz[c] - cb04_.ht[c]);
+      rc = rc + std::fabs(dums_.sst1[c] - soot_.stmf[c]);
+      for (int k = 0; k < shim::LSP - 1; ++k) rc = rc + std::fabs(dumy_.ssp[k][c] - cb04_.fsp[k][c]);
+      cb04_.ht[c] = dumy_.fpz[c];
+      soot_.stmf[c] = dums_.sst1[c];
+      soot_.stnd[c] = dums_.sst2[c];
+      for (int k = 0; k < shim::LSP - 1; ++k) cb04_.fsp[k][c] = dumy_.ssp[k][c];
+    }
+  *resdh = rh / hnorm;
+  *resdc = rc / spnorm;
 }
 
 }  // extern "C"
