@@ -201,7 +201,7 @@ static void clip_closure(void);
  * yys[nj] (field layout, j fastest).  Output of "asm": ekt[n], emu[n], coef[204][n]
  * (/DUMMY/ then /DUM0/), stc[4][n] (STE..STS), zc[4][n] (ZE..ZS); of "spg": isor1, isor2
  * (int64), resdc, resdh, then y[52][n], tk, ht, stmf, stnd, qdot. */
-static int run_assemble(const char *inp, const char *outp, int full) {
+static int run_assemble(const char *inp, const char *outp, int full, int ncalls) {
     if (!spgsolv_) die("this driver has no SPGSOLV (build ref_driver_asm)");
     FILE *f = fopen(inp, "rb");
     if (!f) die("cannot open input");
@@ -262,8 +262,15 @@ static int run_assemble(const char *inp, const char *outp, int full) {
     double relxsp = full ? 0.99 : 1.0, tolrsp = full ? 1e-8 : 1.0, resdc = 0.0;
     double relxh = full ? 0.99 : 1.0, tolrh = full ? 1e-8 : 1.0, resdh = 0.0;
     double sigsp = sc[1], sigh = sc[2];
-    if (full) clip_closure();
-    spgsolv_(&isor1, &relxsp, &tolrsp, &resdc, &isor2, &relxh, &tolrh, &resdh, &sigh, &sigsp);
+    for (int call = 0; call < (full ? ncalls : 1); ++call) { /* chained calls: seconds of each on stderr */
+        if (full) {
+            clip_closure();
+            isor1 = 100; isor2 = 100;
+        }
+        const double t0 = now();
+        spgsolv_(&isor1, &relxsp, &tolrsp, &resdc, &isor2, &relxh, &tolrh, &resdh, &sigh, &sigsp);
+        if (full) fprintf(stderr, "spgsolv call %d: %.6f s, sweeps %d/%d\n", call + 1, now() - t0, isor1, isor2);
+    }
     FILE *fo = fopen(outp, "wb");
     if (!fo) die("cannot open output");
     double *buf = malloc(8 * n);
@@ -296,9 +303,9 @@ static int run_assemble(const char *inp, const char *outp, int full) {
 }
 
 int main(int argc, char **argv) {
-    if (argc < 3) die("usage: ref_driver mech <out> | run <in> <out> | asm <in> <out> | spg <in> <out>");
-    if (!strcmp(argv[1], "asm") && argc >= 4) return run_assemble(argv[2], argv[3], 0);
-    if (!strcmp(argv[1], "spg") && argc >= 4) return run_assemble(argv[2], argv[3], 1);
+    if (argc < 3) die("usage: ref_driver mech <out> | run <in> <out> | asm <in> <out> | spg <in> <out> [ncalls]");
+    if (!strcmp(argv[1], "asm") && argc >= 4) return run_assemble(argv[2], argv[3], 0, 1);
+    if (!strcmp(argv[1], "spg") && argc >= 4) return run_assemble(argv[2], argv[3], 1, argc > 4 ? atoi(argv[4]) : 1);
     if (!strcmp(argv[1], "mech")) {
         double rv[6] = {10.0, 294.0, 1.0133e+05, 0.0, 0.100, 0.233};
         setup(rv);

@@ -37,11 +37,13 @@ def available(driver: str = "ref") -> bool:
 ASM_OUT = ("ekt", "emu", "coef", "stc", "zc")
 
 
-def spgsolv(field, tr, driver="asm", ref_values=(10.0, 294.0, 1.0133e+05, 0.0, 0.100, 0.233)) -> dict:
+def spgsolv(field, tr, driver="asm", ref_values=(10.0, 294.0, 1.0133e+05, 0.0, 0.100, 0.233),
+            ncalls=1) -> dict:
     """The whole of SPGSOLV after main:821-828 (ICHEM = ITHRM = 1; ISOR 100, RELX 0.99,
     TOL 1e-8): driver "asm" runs the flang-built spgsolv.f, "shim" the drop-in's spgsolv_
-    (libunicorn_fshim.so).  Returns isor1, isor2, resdc, resdh and the new y, tk, ht,
-    stmf, stnd, qdot (field layout)."""
+    (libunicorn_fshim.so); ncalls chained calls.  Returns the last call's isor1, isor2,
+    resdc, resdh, the new y, tk, ht, stmf, stnd, qdot (field layout) and the seconds of each
+    call."""
     exe = SHIM_DRIVER if driver == "shim" else ASM_DRIVER
     if not exe.exists():
         raise FileNotFoundError(f"{exe} not built (make -C oracle asm shim)")
@@ -49,13 +51,14 @@ def spgsolv(field, tr, driver="asm", ref_values=(10.0, 294.0, 1.0133e+05, 0.0, 0
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         _write_asm_input(fin, field, tr, ref_values)
-        subprocess.run([str(exe), "spg", fin, fout], check=True, capture_output=True, text=True)
+        p = subprocess.run([str(exe), "spg", fin, fout, str(ncalls)], check=True, capture_output=True, text=True)
         raw = np.fromfile(fout, dtype=np.uint8)
+    calls = [float(l.split()[3]) for l in p.stderr.splitlines() if l.startswith("spgsolv call")]
     cnt = raw[:16].view(np.int64)
     res = raw[16:32].view(np.float64)
     a = raw[32:].view(np.float64)
     out = {"isor1": int(cnt[0]), "isor2": int(cnt[1]), "resdc": float(res[0]), "resdh": float(res[1]),
-           "y": a[:52 * n].reshape(52, n)}
+           "y": a[:52 * n].reshape(52, n), "seconds_per_call": calls}
     o = 52 * n
     for k in ("tk", "ht", "stmf", "stnd", "qdot"):
         out[k] = a[o:o + n]
