@@ -1146,7 +1146,12 @@ __global__ void __launch_bounds__(256) k_asm_ekt(const UcParams* __restrict__ P,
 // (spgsolv.f:1208).  One thread per cell, J fastest across lanes.
 __global__ void __launch_bounds__(256) k_asm_coef(AsmArgs a) {
 #pragma clang fp contract(off)
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-aware order: the hardware deals blocks to the 8 XCDs round-robin (block b to XCD
+  // b % 8), each with its own L2; renumbered so that each XCD takes a contiguous band of
+  // rows, a cell's I-1 and I+1 neighbours are read through the same L2 as the cell itself
+  const int64_t nb = gridDim.x, b = blockIdx.x, x = b % 8, q = nb / 8, r = nb % 8;
+  const int64_t lb = x * q + (x < r ? x : r) + b / 8;
+  const int64_t t = lb * blockDim.x + threadIdx.x;
   const int32_t nj = a.lj - 2;
   if (t >= (int64_t)(a.li - 2) * nj) return;
   const int32_t i = 1 + (int32_t)(t / nj), j = 1 + (int32_t)(t % nj);
