@@ -13,9 +13,12 @@ Cases:
   coupled 25x17 CH4 field (seed 13), soot in 30 % of cells, a block of ISKIP=10 cells
           inside, and the coupling inputs of unicorn_amd.fields.synthetic_coupling (seed 17): E/W/P/Q, STE..STS, ZE..ZS,
           RDHSP inputs, geometry and velocities (stored as cpl_*): 1 and 3 steps
-  assemble the coupled case's field with unicorn_amd.fields.synthetic_transport (seed 19):
-          SPGSOLV's coefficient assembly by the flang-built spgsolv.f (ref_driver_asm),
-          inputs tr_*, outputs asm_* (EKT, EMU, E/W/P/Q, STE..STS, ZE..ZS)
+  assemble the coupled case's field with unicorn_amd.fields.synthetic_transport (seed 19,
+          velocity 0.3, ISYM 1: every face diffusion-dominated): SPGSOLV's coefficient
+          assembly by the flang-built spgsolv.f (ref_driver_asm), inputs tr_*, outputs
+          asm_* (EKT, EMU, E/W/P/Q, STE..STS, ZE..ZS)
+  assemble_conv  the same with seed 21, velocity 5, ISYM 0 (planar): about a third of the
+          faces convection-dominated, so the hybrid scheme's other branch is pinned too
 Each holds the input field and, per step count, the final field, QDOT, QDSP of
 24 interior cells, the sweep counts and residuals per step; plus RC/SOUR of
 those 24 cells after the first CKRATES.
@@ -92,26 +95,28 @@ CASE_SPECS = {
     "c7band": lambda: case("c7band", flame_field(41, 25, "C7H16", 7, preignition_band=0.1), [2]),
     "sooty": lambda: case("sooty", sooty_field(), [1]),
     "coupled": lambda: (lambda h: case("coupled", h, [1, 3], synthetic_coupling(h, seed=17)))(coupled_field()),
-    "assemble": lambda: assemble_case(),
+    "assemble": lambda: assemble_case("assemble", seed=19, velocity=0.3, isym=1),
+    "assemble_conv": lambda: assemble_case("assemble_conv", seed=21, velocity=5.0, isym=0),
 }
 
 ASM_IN = ("vtc", "tmu", "vdf", "stdf", "u", "v", "ygeo", "xxc", "xxs", "yyc", "yys")
 
 
-def assemble_case():
+def assemble_case(name, seed, velocity, isym):
     """SPGSOLV's coefficient assembly (oracle/_ref/ref_driver_asm, the flang-built
-    spgsolv.f) on the coupled case's field with synthetic transport inputs (seed 19):
-    the field, the inputs (tr_*) and the outputs (asm_*)."""
+    spgsolv.f) on the coupled case's field with synthetic transport inputs: the field, the
+    inputs (tr_*) and the outputs (asm_*)."""
     h = coupled_field()
-    tr = synthetic_transport(h, seed=19)
+    tr = synthetic_transport(h, seed=seed, velocity=velocity)
+    tr.isym = isym
     out = refrun.assemble(h, tr)
     arrays = dict(li=h.li, lj=h.lj, in_rho=h.rho, in_y=h.y, in_tk=h.tk, in_ht=h.ht, in_stmf=h.stmf,
                   in_stnd=h.stnd, in_iskip=h.iskip, dt=h.dt, fuel=h.fuel, tr_sigsp=tr.sigsp, tr_sigh=tr.sigh,
                   tr_isym=tr.isym)
     arrays.update({f"tr_{k}": getattr(tr, k) for k in ASM_IN})
     arrays.update({f"asm_{k}": v for k, v in out.items()})
-    np.savez_compressed(OUT / "assemble.npz", **arrays)
-    print("assemble: written", flush=True)
+    np.savez_compressed(OUT / f"{name}.npz", **arrays)
+    print(f"{name}: written", flush=True)
 
 
 def main(only=()):

@@ -20,6 +20,7 @@ def load(name: str):
     return f, z
 
 
+ASM_CASES = ("assemble", "assemble_conv")   # SPGSOLV's assembly by the flang-built spgsolv.f
 ASM_IN = ("vtc", "tmu", "vdf", "stdf", "u", "v", "ygeo", "xxc", "xxs", "yyc", "yys")
 
 

@@ -123,7 +123,8 @@ def test_shim_matches_reference_fixtures(name):
 
 
 @pytest.mark.gpu
-def test_shim_spgsolv_matches_reference():
+@pytest.mark.parametrize("name", ["assemble", "assemble_conv"])
+def test_shim_spgsolv_matches_reference(name):
     """The drop-in's spgsolv_ (coefficient assembly on the device, then the coupled
     chemistry, then SPGSOLV's epilogue) against the flang-built spgsolv.f on the same
     COMMON blocks (tests/golden/assemble.npz's field and transport inputs): sweep counts
@@ -132,7 +133,7 @@ def test_shim_spgsolv_matches_reference():
     from golden_io import load_transport
     if not (refrun.available("shim") and refrun.available("asm")):
         pytest.skip("reference SPGSOLV and shim drivers not built")
-    f0, z = load("assemble")
+    f0, z = load(name)
     tr = load_transport(z)
     ref = refrun.spgsolv(f0.copy(), tr, driver="asm")
     print("shim spgsolv_", ref["isor1"], ref["isor2"], flush=True)

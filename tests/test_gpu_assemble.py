@@ -31,10 +31,11 @@ def gpu_assemble(f, tr):
     return {k: getattr(cp, k).cpu().numpy() for k in ("coef", "stc", "zc", "ekt")}, cp, dev, ctx
 
 
-def test_gpu_assembly_matches_spgsolv_bitwise(torch_cuda):
-    f, z = load("assemble")
+@pytest.mark.parametrize("name", ["assemble", "assemble_conv"])
+def test_gpu_assembly_matches_spgsolv_bitwise(torch_cuda, name):
+    f, z = load(name)
     tr = load_transport(z)
-    print("assemble fixture", f.li, f.lj, flush=True)
+    print("assemble fixture", name, f.li, f.lj, flush=True)
     g, _, _, _ = gpu_assemble(f, tr)
     cells = assembled_cells(f)
     assert np.array_equal(g["ekt"], z["asm_ekt"])

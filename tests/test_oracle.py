@@ -271,14 +271,16 @@ def test_mechanism_table_consistency():
     assert json.dumps(m["tpol"]) == json.dumps([float(1000).hex(), float(5000).hex()])
 
 
-def test_oracle_assembly_reproduces_spgsolv_bitwise():
+@pytest.mark.parametrize("name", ["assemble", "assemble_conv"])
+def test_oracle_assembly_reproduces_spgsolv_bitwise(name):
     """SPGSOLV's coefficient assembly (spgsolv.f:98-1290) restated in the oracle equals the
     flang-built spgsolv.f (tests/golden/assemble.npz) bit for bit: EKT on every cell, the
     E/W/P/Q of the 51 species, STE..STS and ZE..ZS on every assembled cell.  The soot W
     coefficient's comparison with the enthalpy's upwinded value (spgsolv.f:1208) changes
-    STW in most cells of this fixture, so that quirk is pinned too."""
+    STW in most cells of "assemble", so that quirk is pinned too; "assemble_conv" (ISYM 0,
+    strong velocities) has about a third of its faces convection-dominated."""
     from golden_io import assembled_cells, load_transport
-    f, z = load("assemble")
+    f, z = load(name)
     tr = load_transport(z)
     o = pyoracle.assemble(f, tr)
     cells = assembled_cells(f)
