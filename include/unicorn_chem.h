@@ -13,10 +13,17 @@
  *      solveh_   replaces SUBROUTINE SOLVEH(ISOR2,RELXH,TOLRH,SIGSP,HNORM)
  *                                                                 (src/lib/solveh.f:1)
  *
+ *    or, for a build that drops spgsolv.f as a whole,
+ *
+ *      spgsolv_  replaces SUBROUTINE SPGSOLV(ISOR1,RELXSP,TOLRSP,RESDC,
+ *                ISOR2,RELXH,TOLRH,RESDH,SIGH,SIGSP)             (src/lib/spgsolv.f:6)
+ *                with the coefficient assembly on the device too
+ *
  *    They live in libunicorn_fshim.so (unicorn_amd/csrc/fortran_shim.hip),
  *    built for the grid PARAMETER(LI,LJ) of the Fortran build, and read and
  *    write the COMMON blocks /CB01/ /CB02/ /CB03/ /CB04/ /CB06/ /CB08/ /REAC/
- *    /HFORM/ /HRLS/ /DUMD/ /SOOT/ /DUMY/ /DUMS/ /DUMR/ /DUMMY/ /DUM0/ /DUMX/
+ *    /HFORM/ /HRLS/ /DUMD/ /SOOT/ /DUMY/ /DUMS/ /DUMR/ /DUMMY/ /DUM0/ /DUMX/ (and
+ *    /CB07/ for spgsolv_)
  *    exactly as the reference routines do, with or without the transport
  *    coefficients SPGSOLV assembles (E/W/P/Q, ZE..ZS, STE..STS, the RDHSP inputs)
  *    and with the velocities of T(h); everything runs on the GPU (there is no CPU
