@@ -1,6 +1,6 @@
 /* Test infrastructure (oracle): NOT part of the shipped product.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg load this library,
- * and only as the checker.
+ * and only to compare results against.
  *
  * CPU restatement of UNICORN's chemistry step (the CKRATES -> SOLVESP -> SOLVEH
  * chain that SPGSOLV calls, src/lib/spgsolv.f:1293-1298).  uco_step: the

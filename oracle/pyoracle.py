@@ -2,7 +2,7 @@
 
 ctypes binding of oracle/_ref/liboracle.so (oracle/chem_oracle.c), the C
 restatement of the reference chemistry step.  Only tests/, smoke() and the
-bench's cpu_baseline leg use it, as the checker.
+bench's cpu_baseline leg use it, to compare results against.
 """
 from __future__ import annotations
 
