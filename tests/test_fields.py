@@ -43,3 +43,35 @@ def test_synthetic_coupling_flux_scale_is_linear():
     for k in ("coef", "stc", "zc", "u", "v"):
         assert np.array_equal(getattr(a, k), getattr(b, k))
     assert np.all(a.vdf > 0.0) and np.all(a.tmu >= 0.0) and np.all(a.ekt >= 0.0)
+
+
+def _convection_share(f, tr):
+    """Share of (E face, species) pairs where |rho u| exceeds the face conductance
+    (spgsolv.f:233-349), i.e. where the hybrid scheme upwinds."""
+    from unicorn_amd.scales import Scales
+    sc = Scales()
+    lj = f.lj
+    emu = sc.rei * sc.beta3 * f.rho
+    inter = f.interior()
+    ce = inter + lj
+    i = inter // lj
+    rhoue = 0.5 * (f.rho[ce] + f.rho[inter]) * tr.u[ce]
+    tur = sc.rei * (tr.tmu[ce] + tr.tmu[inter]) / tr.sigsp
+    d = (emu[ce] * tr.vdf[:51, ce] + emu[inter] * tr.vdf[:51, inter] + tur) / tr.xxs[i + 1]
+    return float(np.mean(np.abs(rhoue)[None, :] > d))
+
+
+def test_synthetic_transport_regimes():
+    """The transport generator's inputs are positive and shaped for the field; at velocity
+    0.3 every face is diffusion-dominated, at velocity 5 a good share is convection-
+    dominated (the two regimes the assembly fixtures pin)."""
+    from unicorn_amd.fields import flame_field, synthetic_transport
+    f = flame_field(25, 17, "CH4", 13)
+    slow = synthetic_transport(f, seed=19, velocity=0.3)
+    fast = synthetic_transport(f, seed=21, velocity=5.0)
+    for tr in (slow, fast):
+        assert tr.vdf.shape == (52, f.n) and tr.xxc.shape == (f.li,) and tr.yys.shape == (f.lj,)
+        assert (tr.vdf > 0).all() and (tr.tmu >= 0).all() and (tr.vtc >= 0).all() and (tr.stdf >= 0).all()
+        assert np.abs(tr.u).max() > 0 and np.abs(tr.v).max() > 0
+    assert _convection_share(f, slow) == 0.0
+    assert 0.1 < _convection_share(f, fast) < 0.9
