@@ -467,6 +467,31 @@ static double sh_flux(const uco_coupling *cp, const double *rd, const double *ss
     return dt * 0.5 * x;
 }
 
+/* main:1056-1073 (and main:1188-1190, RHO = RHONP): the density of every cell from its
+ * mass fractions and temperature, the last I column extrapolated linearly, the J = 1 row
+ * copied from J = 2. */
+static void density(const uc_scales *S, int li, int lj, const double *y, const double *tk, double *rho) {
+    const int64_t n = (int64_t)li * lj;
+    double *rhonp = malloc(sizeof(double) * (size_t)n);
+    for (int64_t c = 0; c < n; ++c) {
+        double rbar = 0.0;
+        for (int k = 0; k < NSP; ++k) rbar = rbar + y[k * n + c] / S->wm[k];
+        rhonp[c] = S->pback / (S->beta1 * tk[c] * rbar);
+    }
+    for (int64_t c = n - lj; c < n; ++c) rhonp[c] = 2.0 * rhonp[c - lj] - rhonp[c - 2 * lj];
+    for (int64_t c = 0; c < n; c += lj) rhonp[c] = rhonp[c + 1];
+    memcpy(rho, rhonp, sizeof(double) * (size_t)n);
+    free(rhonp);
+}
+
+/* The post-step density on its own (the same function the step ends with). */
+void uco_density(const double *ref, int li, int lj, const double *y, const double *tk, double *rho) {
+    uc_scales S;
+    uc_scales_init(&S, ref[0], ref[1], ref[2], ref[3], ref[4], ref[5], 1);
+    uc_scales_hfo(&S, MT_NASA);
+    density(&S, li, lj, y, tk, rho);
+}
+
 static int step_impl(const uco_params *p, int li, int lj, double *rho, double *y, double *tk, double *ht,
                      double *stmf, double *stnd, const int32_t *iskip, double *qdot, double *qdsp,
                      const uco_coupling *cp, uco_stats *st, double *rc_dump, const int64_t *dump_cells,
@@ -712,17 +737,8 @@ static int step_impl(const uco_params *p, int li, int lj, double *rho, double *y
     }
     st->cell_sweeps = tot;
     st->newton_iters = newton;
-    /* main:1056-1073 */
-    double *rhonp = malloc(sizeof(double) * (size_t)n);
-    for (int64_t c = 0; c < n; ++c) {
-        double rbar = 0.0;
-        for (int k = 0; k < NSP; ++k) rbar = rbar + y[k * n + c] / S.wm[k];
-        rhonp[c] = S.pback / (S.beta1 * tk[c] * rbar);
-    }
-    for (int64_t c = n - lj; c < n; ++c) rhonp[c] = 2.0 * rhonp[c - lj] - rhonp[c - 2 * lj];
-    for (int64_t c = 0; c < n; c += lj) rhonp[c] = rhonp[c + 1];
-    memcpy(rho, rhonp, sizeof(double) * (size_t)n);
-    free(rhonp); free(pos); free(zfrozen); free(rhsz); free(fpz); free(nsw); free(frozen);
+    density(&S, li, lj, y, tk, rho);
+    free(pos); free(zfrozen); free(rhsz); free(fpz); free(nsw); free(frozen);
     free(dl); free(s); free(fa); free(sour); free(rc); free(cells);
     free(sg); free(rd); free(fg);
     return 0;
@@ -864,5 +880,156 @@ int uco_assemble(const double *ref, int li, int lj, const double *rho, const dou
         }
 #undef DF
     free(emu);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------
+ * Mixture-averaged transport properties (main:829-1005, IPROP = 1), every cell:
+ * VMU, VTC (Wilke/Mason-Saxena mixing of Chapman-Enskog species values), VDF of all
+ * 52 species and STDF = 0.2 VDF(N2).  The sub-mixture holds the species 1..51 with
+ * Y >= 1e-3 in index order plus N2; species outside it take the diffusivity of a
+ * trace in the sub-mixture.  Term order, the float32 literals and the reference's
+ * reads are kept: DIFSP's diagonal is never assigned by the reference (zero in its
+ * static storage), so a sub-mixture species whose mole sum excludes nothing
+ * (TMOLE - X <= 1e-10) with no later species in the sub-mixture gets VDF = 0.
+ * ------------------------------------------------------------------------------------ */
+typedef struct {
+    double wmik[NSP][NSP], dwmik[NSP][NSP], wmcd[NSP][NSP], etlj[4][NSP][NSP];
+    double convrc, convrm, convrd;
+} tr_tables;
+
+/* main:449-451 (CONVRC, CONVRM, CONVRD) and main:469-482 (pair tables) */
+static void tr_setup(const uc_scales *S, tr_tables *T) {
+    T->convrc = F32(418.68) * 1.9891e-04 * sqrt(S->tstr / 1000.0 / S->wmstr) / S->acstr;
+    T->convrm = F32(0.1) * 2.6693e-05 * sqrt(1000.0 * S->wmstr * S->tstr) / S->amstr;
+    T->convrd = F32(0.02628) * S->tstr * sqrt(S->tstr) / S->pstr / sqrt(S->wmstr * 1000.0) / S->dstr;
+    for (int i = 0; i < NSP; ++i)
+        for (int j = 0; j < NSP; ++j) {
+            T->wmik[i][j] = pow(S->wm[i] / S->wm[j], 0.25);
+            if (i == j) T->wmik[i][j] = sqrt(S->wm[i]);
+            T->dwmik[i][j] = F32(2.828427125) * sqrt(1.0 + S->wm[i] / S->wm[j]);
+            const double ecdia = 0.5 * (MT_CDIA[i] + MT_CDIA[j]);
+            T->wmcd[i][j] = sqrt((S->wm[i] + S->wm[j]) / 2.0 / S->wm[i] / S->wm[j]) / ecdia / ecdia;
+            const double e1 = sqrt(MT_TLJ[i] * MT_TLJ[j]);
+            T->etlj[0][i][j] = e1;
+            T->etlj[1][i][j] = e1 * e1;
+            T->etlj[2][i][j] = T->etlj[1][i][j] * e1;
+            T->etlj[3][i][j] = T->etlj[2][i][j] * e1;
+        }
+}
+
+/* the reduced collision integral fit of pair (a, b) at TKD1 (main:895-900) */
+static inline double tr_ci11(const tr_tables *T, int a, int b, double tkd1, double tkd2, double tkd3,
+                             double tkd4) {
+    const double tkds = tkd1 / T->etlj[0][a][b];
+    int k = 0;
+    if (tkds >= 2.0) k = 1;
+    if (tkds > 5.0) k = 2;
+    if (tkds > 70.0) k = 3;
+    const double *c = &MT_CI11[5 * k];
+    return c[0] + c[1] * tkds + c[2] * tkd2 / T->etlj[1][a][b] + c[3] * tkd3 / T->etlj[2][a][b]
+         + c[4] * tkd4 / T->etlj[3][a][b];
+}
+
+static void tr_cell(const uc_scales *S, const tr_tables *T, int64_t n, int64_t c, const double *y, double tk,
+                    double *vmu, double *vtc, double *vdf, double *stdf) {
+    const double FLMT = 1.0e-03;
+    int q[NSP];           /* ISPQR: sub-mixture position -> species */
+    int inq[NSP];         /* ISPQ: species -> position + 1, 0 if absent */
+    double x[NSP], atc[NSP], amu[NSP], vdfq[NSP];
+    static double difsp[NSP][NSP]; /* the reference's DIFSP, diagonal never assigned */
+    int ia = 0;
+    double tmass = 0.0, tmole = 0.0;
+    for (int k = 0; k < NSP - 1; ++k) {
+        inq[k] = 0;
+        if (y[k * n + c] < FLMT) continue;
+        q[ia] = k;
+        inq[k] = ia + 1;
+        tmass = tmass + y[k * n + c];
+        x[ia] = y[k * n + c] / S->wm[k];
+        tmole = tmole + x[ia];
+        ++ia;
+    }
+    inq[NSP - 1] = 0;
+    int lspt = ia + 1;
+    if (lspt >= NSP) lspt = NSP;
+    q[lspt - 1] = NSP - 1;
+    x[lspt - 1] = (1.0 - tmass) / S->wm[NSP - 1];
+    tmole = tmole + x[lspt - 1];
+    const double tkrt = sqrt(tk);
+    const double tkd1 = tk * S->tstr, tkd2 = tkd1 * tkd1, tkd3 = tkd2 * tkd1, tkd4 = tkd3 * tkd1,
+                 tkd5 = tkd4 * tkd1;
+    int ipl1 = 0, ipl2 = 0;
+    if (tkd1 >= MT_TPOL1) { ipl1 = 7; ipl2 = 6; }
+    const double tkbp = tk * tkrt / S->pback;
+    for (int a = 0; a < lspt; ++a) {
+        const int ka = q[a];
+        const double *p = &MT_NASA[14 * ka + ipl1];
+        const double euken = F32(0.115) + F32(0.354) * (p[0] + p[1] * tkd1 + p[2] * tkd2 + p[3] * tkd3
+                                                       + p[4] * tkd4);
+        const double *ci = &MT_CISP[12 * ka + ipl2];
+        const double siome = MT_CDIA[ka] * MT_CDIA[ka] * (ci[0] + ci[1] * tkd1 + ci[2] * tkd2 + ci[3] * tkd3
+                                                          + ci[4] * tkd4 + ci[5] * tkd5);
+        atc[a] = T->convrc * euken * tkrt / T->wmik[ka][ka] / siome;
+        amu[a] = T->convrm * tkrt * T->wmik[ka][ka] / siome;
+        for (int b = a + 1; b < lspt; ++b) {
+            const int kb = q[b];
+            const double ci11 = tr_ci11(T, ka, kb, tkd1, tkd2, tkd3, tkd4);
+            difsp[a][b] = T->wmcd[ka][kb] * tkbp / ci11;
+            difsp[b][a] = T->wmcd[kb][ka] * tkbp / ci11;
+        }
+    }
+    double vt = 0.0, vm = 0.0;
+    for (int a = 0; a < lspt; ++a) {
+        const int ka = q[a];
+        double phik = 0.0, phdik = 0.0, efdik = 0.0;
+        for (int b = 0; b < lspt; ++b) {
+            if (b == a) continue;
+            const int kb = q[b];
+            const double t1 = 1.0 + sqrt(atc[a] / atc[b]) * T->wmik[ka][kb];
+            phik = phik + x[b] * (t1 * t1) / T->dwmik[ka][kb];
+            const double t2 = 1.0 + sqrt(amu[a] / amu[b]) * T->wmik[ka][kb];
+            phdik = phdik + x[b] * (t2 * t2) / T->dwmik[ka][kb];
+            efdik = efdik + x[b] / difsp[a][b];
+        }
+        vt = vt + atc[a] * x[a] / (x[a] + F32(1.065) * phik);
+        vm = vm + amu[a] * x[a] / (x[a] + phdik);
+        const double aa = tmole - x[a];
+        if (aa <= 1.0e-10)
+            vdfq[a] = T->convrd * difsp[a][lspt - 1];
+        else
+            vdfq[a] = T->convrd * (tmole - x[a]) / efdik;
+    }
+    vtc[c] = vt;
+    vmu[c] = vm;
+    for (int k = 0; k < NSP; ++k) {
+        const int is = inq[k];
+        if (is == 0) {
+            double efd = 0.0;
+            for (int b = 0; b < lspt; ++b) {
+                const int kb = q[b];
+                const double ci11 = tr_ci11(T, k, kb, tkd1, tkd2, tkd3, tkd4);
+                efd = efd + x[b] / (T->wmcd[k][kb] * tkbp / ci11);
+            }
+            vdf[k * n + c] = T->convrd * tmole / efd;
+        } else {
+            vdf[k * n + c] = vdfq[is - 1];
+        }
+    }
+    stdf[c] = F32(0.2) * vdf[(NSP - 1) * n + c];
+}
+
+/* Transport properties of n cells (main:843-1005).  y[52][n]: mass fractions after
+ * main:821-828; tk: T/TSTR.  Serial (the reference's DIFSP persists across cells). */
+int uco_transport(const double *ref, int64_t n, const double *y, const double *tk, double *vmu, double *vtc,
+                  double *vdf, double *stdf) {
+    uc_scales S;
+    uc_scales_init(&S, ref[0], ref[1], ref[2], ref[3], ref[4], ref[5], 1);
+    uc_scales_hfo(&S, MT_NASA);
+    tr_tables *T = malloc(sizeof *T);
+    if (!T) return -1;
+    tr_setup(&S, T);
+    for (int64_t c = 0; c < n; ++c) tr_cell(&S, T, n, c, y, tk[c], vmu, vtc, vdf, stdf);
+    free(T);
     return 0;
 }

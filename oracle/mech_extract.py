@@ -1,5 +1,13 @@
-"""Extract the San Diego 52-species mechanism of UNICORN's chemistry step into a
-data table (``sd52.json``) that the HIP kernels and the CPU oracle are built from.
+"""Test infrastructure (oracle): the generator of two data tables the product is
+built from, run by hand where the reference tree is present; its outputs are
+committed under unicorn_amd/mech/.
+
+1. The San Diego 52-species mechanism of UNICORN's chemistry step (``sd52.json``),
+   which the HIP kernels and the CPU oracle are built from.
+2. The transport-property tables of the main program (``transport.json``): the
+   Lennard-Jones diameters CDIA and well depths TLJ (main:97-108 DATA), the
+   collision-integral fits CI11 (main:109-113 DATA) and the conductivity/viscosity
+   integral fits CISP of /CB08/ (src/lib/DATA.f), read by main:829-1005.
 
 The reference encodes the mechanism partly as data (``src/lib/DATA.f``: Arrhenius
 and fall-off parameters, NASA-7 polynomials) and partly as straight-line code:
@@ -23,7 +31,7 @@ Numeric constants (A, b, E/R, fall-off parameters, NASA-7 tables) are NOT taken
 from here: they are dumped by the flang-compiled reference itself (oracle/ref
 driver) so that float32-widened literals are exact.
 
-Usage:  python -m unicorn_amd.mech.extract [/root/reference] [out.json]
+Usage:  python oracle/mech_extract.py --write   (both tables; needs oracle/_ref/ref_driver)
 """
 from __future__ import annotations
 
@@ -572,15 +580,45 @@ WSP = [2.016, 32.00, 1.008, 16.00, 17.01, 18.02, 33.01, 34.01, 28.01, 44.01, 29.
        42.08, 44.10, 43.09, 43.09, 54.09, 29.02, 68.12, 100.2, 56.11, 70.14, 39.95, 4.003, 28.01]
 
 
+def main_data(name: str, ref_root="/root/reference") -> list:
+    """The values of one DATA statement of the main program (all REAL*8 literals with
+    a D exponent, read exactly)."""
+    prog = Path(ref_root) / "src" / "unicornd-mild-sd-5.f"
+    for _, _, code in statements(prog):
+        m = re.match(r"^DATA%s/(.*)/$" % re.escape(name), code)
+        if m:
+            vals = [v.replace("D", "E") for v in m.group(1).split(",")]
+            return [float(v) for v in vals]
+    raise MechError(f"DATA {name} not found")
+
+
+def transport_table(dump, ref_root="/root/reference") -> dict:
+    """CDIA (52), TLJ (52), CI11 (4x5, the DATA's column-major order turned into rows
+    K=1..4) and CISP (52x12, from the flang build's /CB08/), all as hex floats."""
+    cdia, tlj, ci = main_data("CDIA", ref_root), main_data("TLJ", ref_root), main_data("CI11", ref_root)
+    check(len(cdia) == 52 and len(tlj) == 52 and len(ci) == 20, "transport DATA sizes")
+    ci11 = [[ci[k + 4 * m] for m in range(5)] for k in range(4)]      # CI11(K, M), K fastest in DATA
+    return {"about": "transport-property tables of UNICORN's main program: CDIA, TLJ "
+                     "(src/unicornd-mild-sd-5.f DATA), CI11 (K=1..4 rows, M=1..5 columns), "
+                     "CISP(12,52) of /CB08/ (src/lib/DATA.f) as the flang build holds it",
+            "cdia": [h(v) for v in cdia], "tlj": [h(v) for v in tlj],
+            "ci11": [[h(v) for v in row] for row in ci11],
+            "cisp": [[h(v) for v in dump["cisp"][k]] for k in range(52)]}
+
+
 if __name__ == "__main__" and len(sys.argv) > 1 and sys.argv[1] == "--write":
     import subprocess
     import tempfile
     here = Path(__file__).resolve().parent
-    drv = here.parents[1] / "oracle" / "_ref" / "ref_driver"
+    drv = here / "_ref" / "ref_driver"
+    mech = here.parent / "unicorn_amd" / "mech"
     with tempfile.TemporaryDirectory() as td:
         subprocess.run([str(drv), "mech", f"{td}/mech.bin"], check=True)
         dump = read_mech_dump(f"{td}/mech.bin")
     table = to_table(build(), dump)
-    out = here / "sd52.json"
+    out = mech / "sd52.json"
     out.write_text(json.dumps(table, separators=(",", ":")))
     print("wrote", out, "reactions", len(table["reactions"]), "wm_check", table["wm_check"])
+    out = mech / "transport.json"
+    out.write_text(json.dumps(transport_table(dump), indent=0))
+    print("wrote", out)

@@ -71,3 +71,28 @@ def test_scales_match_python_restatement():
     assert s.wmstr == p.wmstr and s.rstr == p.rstr and s.beta1 == p.beta1 and s.pback == p.pback
     assert np.array_equal(np.array(s.wm), p.wm)
     assert np.allclose(np.array(s.hfo), p.hfo, rtol=2e-16, atol=0)
+
+
+def test_device_field_refuses_wrong_shapes_before_any_allocation():
+    """A field whose arrays do not have the (li*lj,) / (52, li*lj) shapes the kernels
+    index is refused on the host; float32 arrays are widened to float64."""
+    from unicorn_amd.fields import flame_field
+    f = flame_field(9, 7, "CH4", seed=1)
+    ok = chem.checked_arrays(f)
+    assert all(a.flags.c_contiguous for a in ok.values())
+    assert ok["y"].dtype == np.float64 and ok["iskip"].dtype == np.int32
+    g = f.copy()
+    g.rho = g.rho.astype(np.float32)
+    assert chem.checked_arrays(g)["rho"].dtype == np.float64
+    for name, bad in (("y", f.y.T.copy()), ("y", f.y[:51]), ("rho", f.rho[:-1]),
+                      ("tk", f.tk.reshape(f.li, f.lj)), ("iskip", f.iskip.astype(np.float64))):
+        g = f.copy()
+        setattr(g, name, bad)
+        with pytest.raises(ValueError, match=name):
+            chem.checked_arrays(g)
+        with pytest.raises(ValueError, match=name):
+            chem.DeviceField(g, device="cpu")   # refused before torch allocates anything
+    g = f.copy()
+    g.li = 2
+    with pytest.raises(ValueError, match="at least 3"):
+        chem.checked_arrays(g)

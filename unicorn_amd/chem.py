@@ -167,16 +167,42 @@ class StepStats:
         return cls(*(getattr(s, n) for n, _ in UcStats._fields_))
 
 
+def checked_arrays(f: ChemField) -> dict:
+    """The field's arrays as contiguous float64 (int32 for ISKIP) host arrays, after
+    checking that every one has the shape the kernels index: (li*lj,) per variable and
+    (52, li*lj) for the species.  Raises ValueError before anything is allocated on the
+    device, since the kernels read li*lj (x 52) elements from each pointer."""
+    if int(f.li) < 3 or int(f.lj) < 3:
+        raise ValueError(f"field {f.li}x{f.lj}: LI and LJ must be at least 3")
+    n = int(f.li) * int(f.lj)
+    out = {}
+    for name, shape in (("rho", (n,)), ("y", (52, n)), ("tk", (n,)), ("ht", (n,)),
+                        ("stmf", (n,)), ("stnd", (n,)), ("iskip", (n,))):
+        a = np.asarray(getattr(f, name))
+        if a.shape != shape:
+            raise ValueError(f"field {f.li}x{f.lj}: {name} has shape {a.shape}, expected {shape}")
+        if name == "iskip":
+            if not np.issubdtype(a.dtype, np.integer):
+                raise ValueError(f"field: iskip has dtype {a.dtype}, expected an integer type")
+            out[name] = np.ascontiguousarray(a, dtype=np.int32)
+        else:
+            if not (np.issubdtype(a.dtype, np.floating) or np.issubdtype(a.dtype, np.integer)):
+                raise ValueError(f"field: {name} has dtype {a.dtype}, expected a real type")
+            out[name] = np.ascontiguousarray(a, dtype=np.float64)
+    return out
+
+
 class DeviceField:
     """A ChemField resident in device memory (torch tensors on the current GPU)."""
 
     def __init__(self, f: ChemField, device="cuda"):
+        host = checked_arrays(f)
         import torch
-        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+        t = lambda a: torch.from_numpy(a).to(device)  # noqa: E731
         self.li, self.lj, self.dt, self.fuel = f.li, f.lj, f.dt, f.fuel
-        self.rho, self.y, self.tk, self.ht = t(f.rho), t(f.y), t(f.tk), t(f.ht)
-        self.stmf, self.stnd = t(f.stmf), t(f.stnd)
-        self.iskip = t(f.iskip.astype(np.int32))
+        self.rho, self.y, self.tk, self.ht = t(host["rho"]), t(host["y"]), t(host["tk"]), t(host["ht"])
+        self.stmf, self.stnd = t(host["stmf"]), t(host["stnd"])
+        self.iskip = t(host["iskip"])
         self.qdot = torch.zeros(f.n, dtype=torch.float64, device=device)
         self.qdsp = torch.zeros((52, f.n), dtype=torch.float64, device=device)
 
@@ -354,11 +380,11 @@ class ChemStep:
 
     def step_host(self, f: ChemField, ctl: Controls) -> tuple[StepStats, np.ndarray, np.ndarray]:
         """Advance a host field (copied to and from the GPU); returns stats, QDOT, QDSP."""
+        h = checked_arrays(f)
         for name in ("rho", "y", "tk", "ht", "stmf", "stnd"):
-            a = getattr(f, name)
-            if a.dtype != np.float64 or not a.flags.c_contiguous:
-                setattr(f, name, np.ascontiguousarray(a, dtype=np.float64))
-        isk = np.ascontiguousarray(f.iskip, dtype=np.int32)
+            if h[name] is not getattr(f, name):
+                setattr(f, name, h[name])   # the step writes the caller's field in place
+        isk = h["iskip"]
         qdot = np.zeros(f.n)
         qdsp = np.zeros((52, f.n))
         p = lambda a: a.ctypes.data  # noqa: E731
