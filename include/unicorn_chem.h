@@ -221,6 +221,16 @@ int uc_autotune(uc_context *ctx, const uc_field *fields, const uc_controls *ctls
                 int32_t ngrids, void *stream, uc_tuning *chosen);
 int uc_set_tuning(uc_context *ctx, const uc_tuning *tuning);
 
+/* Mixture-averaged transport properties of every cell (replaces the inline block
+ * src/unicornd-mild-sd-5.f:843-1005 of PROGRAM UNICORN with IPROP = 1, whose outputs are
+ * /CB07/ VMU, VTC, VDFSP and /SOOT/ STDF): vmu[n], vtc[n], vdf[52][n] and stdf[n] from
+ * field->y (after main:821-828) and field->tk, in the reference's term order and
+ * float32 literals, equal bit for bit to the reference.  TMU (k-epsilon runs,
+ * main:1006-1010) is not part of it.  The outputs feed uc_transport.  Asynchronous on
+ * `stream`; the context builds its pair tables (main:469-482) at the first call. */
+int uc_transport_props(uc_context *ctx, const uc_field *field, double *vmu, double *vtc, double *vdf,
+                       double *stdf, void *stream);
+
 /* The ISKIP contents of the grids changed in place: rebuild the cell list at the
  * next step. */
 int uc_cells_changed(uc_context *ctx);

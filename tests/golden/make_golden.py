@@ -19,6 +19,11 @@ Cases:
           asm_* (EKT, EMU, E/W/P/Q, STE..STS, ZE..ZS)
   assemble_conv  the same with seed 21, velocity 5, ISYM 0 (planar): about a third of the
           faces convection-dominated, so the hybrid scheme's other branch is pinned too
+  transport  the transport properties PROGRAM UNICORN itself computed (main:829-1005, /CB07/
+          at its first CKRATES call, dumped by oracle/_ref/unicorn_spy): 600 cells of the
+          sample deck's cold start and 2400 of a restart from a 711x131 C7H16 flame field
+          with a pre-ignition band (seed 7); the mass fractions and T they were computed
+          from (tr_y, tr_tk) and VMU, VTC, VDF, STDF
 Each holds the input field and, per step count, the final field, QDOT, QDSP of
 24 interior cells, the sweep counts and residuals per step; plus RC/SOUR of
 those 24 cells after the first CKRATES.
@@ -90,6 +95,32 @@ def coupled_field():
     return h
 
 
+def transport_case():
+    """Sampled cells of the program's own transport properties (oracle/progrun.py)."""
+    import tempfile
+    from oracle import progrun as P
+    from unicorn_amd import chem
+    parts = []
+    with tempfile.TemporaryDirectory() as td:
+        for tag, seed, m in (("cold", None, 600), ("restart", 7, 2400)):
+            d = Path(td) / tag
+            (d / "dump").mkdir(parents=True)
+            if seed is None:
+                P.write_deck(d, itend=1)
+            else:
+                P.write_deck(d, itend=1, iread=2, infuel=11)
+                P.flame_restart(d, flame_field(P.LI, P.LJ, "C7H16", seed, preignition_band=0.10),
+                                chem.scales_from_card())
+            P.run("unicorn_spy", d, env={"UC_SPY_DIR": str(d / "dump"), "UC_SPY_STOP": "1"})
+            D = P.Dump(d / "dump")
+            cells = np.sort(np.random.default_rng(99).choice(P.LE, size=m, replace=False))
+            parts.append({k: (D.arr("rates1", k)[..., cells]) for k in ("fsp", "tk", "vmu", "vtc", "vdf", "stdf")})
+    cat = {k: np.concatenate([p[k] for p in parts], axis=-1) for k in parts[0]}
+    np.savez_compressed(OUT / "transport.npz", tr_y=cat["fsp"], tr_tk=cat["tk"], vmu=cat["vmu"], vtc=cat["vtc"],
+                        vdf=cat["vdf"], stdf=cat["stdf"])
+    print("transport: written", cat["tk"].size, "cells", (OUT / "transport.npz").stat().st_size, "bytes")
+
+
 CASE_SPECS = {
     "cfg0": lambda: case("cfg0", flame_field(61, 41, "CH4", 0), [1, 3]),
     "c7band": lambda: case("c7band", flame_field(41, 25, "C7H16", 7, preignition_band=0.1), [2]),
@@ -97,6 +128,7 @@ CASE_SPECS = {
     "coupled": lambda: (lambda h: case("coupled", h, [1, 3], synthetic_coupling(h, seed=17)))(coupled_field()),
     "assemble": lambda: assemble_case("assemble", seed=19, velocity=0.3, isym=1),
     "assemble_conv": lambda: assemble_case("assemble_conv", seed=21, velocity=5.0, isym=0),
+    "transport": transport_case,
 }
 
 ASM_IN = ("vtc", "tmu", "vdf", "stdf", "u", "v", "ygeo", "xxc", "xxs", "yyc", "yys")

@@ -69,7 +69,8 @@ class UcTransport(C.Structure):
 
 EXPORTS = ("uc_abi_version", "uc_last_error", "uc_scales_from_card", "uc_create", "uc_create_batch",
            "uc_destroy", "uc_step", "uc_step_host", "uc_step_batch", "uc_autotune", "uc_set_tuning",
-           "uc_get_tuning", "uc_cells_changed", "uc_clip", "uc_set_clip", "uc_step_coupled", "uc_assemble")
+           "uc_get_tuning", "uc_cells_changed", "uc_clip", "uc_set_clip", "uc_step_coupled", "uc_assemble",
+           "uc_transport_props")
 
 _lib = None
 
@@ -114,6 +115,7 @@ def library() -> C.CDLL:
                                     C.c_void_p, C.POINTER(UcStats)]
     lib.uc_assemble.argtypes = [C.c_void_p, C.POINTER(UcField), C.POINTER(UcControls), C.POINTER(UcTransport),
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.uc_transport_props.argtypes = [C.c_void_p, C.POINTER(UcField)] + [C.c_void_p] * 5
     _lib = lib
     return lib
 
@@ -336,6 +338,20 @@ class ChemStep:
         _check(self.lib.uc_assemble(self.h, C.byref(fc), C.byref(cc), C.byref(tc), cp.coef.data_ptr(),
                                     cp.stc.data_ptr(), cp.zc.data_ptr(), cp.ekt.data_ptr(), C.c_void_p(stream)))
         return cp
+
+    def transport_props(self, f: DeviceField, stream=None) -> dict:
+        """The mixture-averaged transport properties of every cell (uc_transport_props,
+        main:843-1005): new device tensors vmu (n,), vtc (n,), vdf (52, n), stdf (n,)."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        n = f.rho.numel()
+        z = lambda *shape: torch.empty(shape, dtype=torch.float64, device=f.rho.device)  # noqa: E731
+        out = {"vmu": z(n), "vtc": z(n), "vdf": z(52, n), "stdf": z(n)}
+        fc = f.c()
+        _check(self.lib.uc_transport_props(self.h, C.byref(fc), out["vmu"].data_ptr(), out["vtc"].data_ptr(),
+                                           out["vdf"].data_ptr(), out["stdf"].data_ptr(), C.c_void_p(stream)))
+        return out
 
     def step_batch(self, fields: list[DeviceField], ctls: list[Controls], stream=None) -> list[StepStats]:
         """Advance several device-resident grids by one chemistry step each, in one batch."""

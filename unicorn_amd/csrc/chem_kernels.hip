@@ -1029,6 +1029,8 @@ struct SegGraph {
 };
 }  // namespace
 
+#include "transport.inc"
+
 struct uc_context {
   uc_scales sc{};
   bool clip = true;        // uc_step* start with main:821-828 (uc_set_clip)
@@ -1093,6 +1095,7 @@ struct uc_context {
   int32_t* cpl_dfirst = nullptr;  // [ndiag+1] first list position of each diagonal
   int64_t cpl_dfirst_cap = 0;
   int32_t last_nh = 0;          // previous coupled step's enthalpy sweep count
+  TrTables* tr = nullptr;       // transport-property tables (built at first use)
 };
 
 namespace {
@@ -1375,6 +1378,26 @@ int uc_assemble(uc_context* ctx, const uc_field* f, const uc_controls* ctl, cons
   k_asm_ekt<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ctx->dp, a);
   const int64_t ni = (int64_t)(f->li - 2) * (f->lj - 2);
   k_asm_coef<<<(unsigned)((ni + 255) / 256), 256, 0, s>>>(a);
+  HIPCHK(hipGetLastError());
+  return UC_OK;
+}
+
+int uc_transport_props(uc_context* ctx, const uc_field* f, double* vmu, double* vtc, double* vdf, double* stdf,
+                       void* stream) {
+  if (!ctx || !f) return fail(UC_E_ARG, "uc_transport_props: null argument");
+  if (f->li < 3 || f->lj < 3) return fail(UC_E_ARG, "uc_transport_props: grid must be at least 3 x 3");
+  const int64_t n = (int64_t)f->li * f->lj;
+  if (n > ctx->max_cells) return fail(UC_E_ARG, "uc_transport_props: the grid does not fit the context");
+  if (!f->y || !f->tk) return fail(UC_E_ARG, "uc_transport_props: null field array (y and tk are read)");
+  if (!vmu || !vtc || !vdf || !stdf) return fail(UC_E_ARG, "uc_transport_props: null output array");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (!ctx->tr) {
+    std::unique_ptr<TrTables> h(new TrTables);
+    fill_transport_tables(ctx->sc, *h);
+    HIPCHK(hipMalloc(&ctx->tr, sizeof(TrTables)));
+    HIPCHK(hipMemcpy(ctx->tr, h.get(), sizeof(TrTables), hipMemcpyHostToDevice));
+  }
+  k_transport<<<blocks_for(n, 64), 64, 0, (hipStream_t)stream>>>(ctx->tr, f->y, f->tk, n, vmu, vtc, vdf, stdf);
   HIPCHK(hipGetLastError());
   return UC_OK;
 }

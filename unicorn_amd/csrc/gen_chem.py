@@ -637,6 +637,22 @@ def emit_host_tables(t, L):
     L.append("")
 
 
+def emit_transport_tables(L):
+    """Host tables of the transport properties (main:829-1005): CDIA, TLJ (main DATA),
+    CI11 rows K=1..4, CISP(12,52) of /CB08/, from unicorn_amd/mech/transport.json."""
+    tr = json.loads((HERE.parent / "mech" / "transport.json").read_text())
+    L.append("// host tables of the transport properties (main:97-113 DATA; /CB08/ CISP, src/lib/DATA.f)")
+    L.append("static const double uc_tr_cdia[52] = {" + ", ".join(hx(v) for v in tr["cdia"]) + "};")
+    L.append("static const double uc_tr_tlj[52] = {" + ", ".join(hx(v) for v in tr["tlj"]) + "};")
+    L.append("static const double uc_tr_ci11[4][5] = {"
+             + ", ".join("{" + ", ".join(hx(v) for v in row) + "}" for row in tr["ci11"]) + "};")
+    L.append("static const double uc_tr_cisp[52][12] = {")
+    for row in tr["cisp"]:
+        L.append("  {" + ", ".join(hx(v) for v in row) + "},")
+    L.append("};")
+    L.append("")
+
+
 def main(out=None):
     t = load()
     L = ["// generated by unicorn_amd/csrc/gen_chem.py from unicorn_amd/mech/sd52.json -- do not edit",
@@ -656,6 +672,7 @@ def main(out=None):
          "#define UC_NBR 2000",
          f"#define UC_CPL_SLOTS {CPL_SLOTS}", ""]
     emit_host_tables(t, L)
+    emit_transport_tables(L)
     emit_rates(t, L)
     emit_soot(L)
     emit_sweep(t, L)
