@@ -98,7 +98,13 @@ class StepLine:
     isor1: int
 
 
-_STEP = re.compile(r"^\s*(\S+)\(\s*(\d+),\s*(\d+),\s*(\d+)\)\s+(\S+)\s+(\S+)\(\s*(\d+)\)\s+(\S+)\(\s*(\d+)\)")
+# sweep counts are I2 fields: a count above 99 prints as ** (read here as -1)
+_STEP = re.compile(r"^\s*(\S+)\(\s*(\d+),\s*(\d+),\s*(\d+)\)\s+(\S+)\s+(\S+)\(\s*(\d+|\*\*)\)\s+(\S+)\("
+                   r"\s*(\d+|\*\*)\)")
+
+
+def _count(v: str) -> int:
+    return -1 if v == "**" else int(v)
 
 
 def step_lines(stdout: str) -> list:
@@ -108,7 +114,7 @@ def step_lines(stdout: str) -> list:
         if m:
             g = m.groups()
             out.append(StepLine(float(g[0]), (int(g[1]), int(g[2]), int(g[3])), float(g[4]), float(g[5]),
-                                int(g[6]), float(g[7]), int(g[8])))
+                                _count(g[6]), float(g[7]), _count(g[8])))
     return out
 
 

@@ -4,10 +4,12 @@ spgsolv.f's three INCLUDEs, src/lib/spgsolv.f:2-4, resolved to nothing), against
 same program built all flang (oracle/_ref/unicorn_ref), run side by side from the same
 input deck and restart file.
 
-Two runs of the reference's sample deck (docs/input.uni, edited to 2 time steps):
+Runs of the reference's sample deck (docs/input.uni, edited to 2 time steps):
 * from its own cold start (H2 fuel option, the program's inflow conditions);
 * restarted (IREAD = 2) from a seeded 711x131 C7H16 flame field with a pre-ignition
-  band written in PRNTDK's format.
+  band written in PRNTDK's format;
+* both again with ICHEM = 0 (a thermal run without chemistry): SPGSOLV then calls SOLVEH
+  alone (spgsolv.f:1291-1298), which reads the rate coefficients /DUMR/ holds.
 Checked: the per-step residual table the program prints (main:1215-1220) -- the species
 and enthalpy sweep counts exactly, the residuals to their 2-digit print precision --
 and every value of the flame file (FLAMEA.DATA: X, Y, RHO, Y_k, U, V, W, P, QDOT, T,
@@ -43,19 +45,19 @@ def close(a, b):
     return np.abs(a - b) <= RTOL * np.abs(b) + ATOL
 
 
-@pytest.mark.parametrize("case", ["cold", "restart"])
-def test_program_with_the_drop_in_matches_the_reference_program(case, tmp_path):
+@pytest.mark.parametrize("case,ichem", [("cold", 1), ("restart", 1), ("cold", 0), ("restart", 0)])
+def test_program_with_the_drop_in_matches_the_reference_program(case, ichem, tmp_path):
     runs = {}
     for prog in ("unicorn_ref", "unicorn_shim"):
         d = tmp_path / prog
         d.mkdir()
         if case == "cold":
-            P.write_deck(d, itend=2)
+            P.write_deck(d, itend=2, ichem=ichem)
         else:
-            P.write_deck(d, itend=2, iread=2, infuel=11)
+            P.write_deck(d, itend=2, iread=2, infuel=11, ichem=ichem)
             P.flame_restart(d, flame_field(P.LI, P.LJ, "C7H16", seed=7, preignition_band=0.10),
                             chem.scales_from_card())
-        print(f"{case}: running {prog} (2 steps, 711x131)", flush=True)
+        print(f"{case}, ICHEM={ichem}: running {prog} (2 steps, 711x131)", flush=True)
         runs[prog] = (d, P.run(prog, d, timeout=600))
     (dr, out_r), (ds, out_s) = runs["unicorn_ref"], runs["unicorn_shim"]
     lr, ls = P.step_lines(out_r), P.step_lines(out_s)

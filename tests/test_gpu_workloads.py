@@ -1,0 +1,183 @@
+"""The benchmark workloads themselves, at their full sizes, on the GPU.
+
+* 1001 x 401 C7H16 field with a 10 % pre-ignition band, dt = 1e-5 s, 20 chained steps:
+  closure, positivity and bounds on every cell after the last step, and a seeded 2 %
+  sample of the cells re-run through all 20 steps by the oracle at the GPU's per-step
+  global sweep counts (cells are independent in the chemistry-only contract, so the
+  sample's chain is the field's chain restricted to those cells): state to the parity
+  bar, the last step's QDOT/QDSP per value.
+* 256 grids of uneven size (31 x 21 .. 301 x 151, about 2.1 M cells), fuel and dt in
+  one batched step after the autotuner: invariants on every cell of every grid, and the
+  oracle on the largest, the smallest and two more grids with their own global stops
+  (sweep counts exact).
+* The learned surrogate trained from scratch at its full size (20 M pairs from the FP64
+  step, batch 8192, 30 epochs, 5 % held out): the held-out loss and what the network
+  gains over frozen chemistry.
+* The gated surrogate with FP64 fallback over 200 chained steps on the 1001 x 401 field:
+  fallback rate, errors against the all-FP64 run, sum-of-Y drift.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from golden_io import ABS_Y, REL_RATE, REL_T, REL_Y
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def check_invariants(got, cells, label):
+    y = got.y[:, cells]
+    assert np.all(y >= 0.0), label
+    assert np.allclose(y.sum(axis=0) + got.stmf[cells], 1.0, rtol=0, atol=1e-13), label
+    T = got.tk[cells] * 294.0
+    assert T.min() >= 293.0 - 1e-9 and T.max() <= 5000.0, label
+    assert np.isfinite(got.rho[cells]).all() and (got.rho[cells] > 0).all(), label
+
+
+def close_state(a, b, cells, label):
+    ya, yb = a.y[:, cells], b.y[:, cells]
+    bad = np.abs(ya - yb) > REL_Y * np.abs(yb) + ABS_Y
+    assert not bad.any(), (label, int(bad.sum()))
+    for k in ("tk", "ht", "rho"):
+        x, z = getattr(a, k)[cells], getattr(b, k)[cells]
+        assert np.all(np.abs(x - z) <= REL_T * np.abs(z)), (label, k)
+
+
+def test_gpu_1001x401_twenty_chained_steps_sample_matches_oracle(torch_cuda):
+    from unicorn_amd.chem import ChemStep, Controls, DeviceField
+    from unicorn_amd.fields import config1
+    f0 = config1()
+    dev = DeviceField(f0)
+    ctx = ChemStep(f0.n)
+    print("1001x401 C7H16: 20 chained steps on the GPU", flush=True)
+    stats = [ctx.step(dev, Controls(dt=f0.dt)) for _ in range(20)]
+    torch_cuda.cuda.synchronize()
+    got = dev.to_host()
+    inter = f0.interior()
+    check_invariants(got, inter, "after 20 steps")
+    rng = np.random.default_rng(20)
+    cells = np.sort(rng.choice(inter, size=len(inter) // 50, replace=False))
+    sub = f0.take(cells)
+    print(f"oracle: {cells.size} cells through 20 steps at the GPU's counts "
+          f"{[s.isor1 for s in stats]} / {[s.isor2 for s in stats]}", flush=True)
+    for s in stats:
+        out = pyoracle.step(sub, isor1=min(s.isor1, 100), tolrsp=-1.0, isor2=min(s.isor2, 100), tolrh=-1.0)
+    si = sub.active()
+    assert len(si) == len(cells)
+    got_sub = got.take(cells)
+    close_state(got_sub, sub, si, "2% sample after 20 steps")
+    # the last step's production rates, per value, at every sampled cell
+    qd = dev.qdot.cpu().numpy()[cells]
+    qs = dev.qdsp.cpu().numpy()[:, cells]
+    rq, rs = out.qdot[si], out.qdsp[:, si]
+    scale = np.abs(rs).max(axis=1, keepdims=True)
+    assert np.all(np.abs(qd - rq) <= REL_RATE * np.abs(rq).max())
+    assert np.all(np.abs(qs - rs) <= REL_RATE * scale)
+
+
+def test_gpu_256_grids_one_batched_step(torch_cuda):
+    from unicorn_amd.chem import ChemStep, Controls, DeviceField
+    from unicorn_amd.fields import batch_grids
+    grids = batch_grids(256, seed=2)
+    total = sum(g.n for g in grids)
+    print(f"256 grids, {total} cells: autotune, then one batched step", flush=True)
+    assert 1.8e6 < total < 2.5e6
+    devs = [DeviceField(g) for g in grids]
+    ctls = [Controls(dt=g.dt) for g in grids]
+    ctx = ChemStep(total, max_grids=len(grids))
+    t = ctx.autotune(devs, ctls)
+    print("tuning", t, flush=True)
+    stats = ctx.step_batch(devs, ctls)
+    torch_cuda.cuda.synchronize()
+    sizes = np.array([g.n for g in grids])
+    pick = sorted({int(sizes.argmax()), int(sizes.argmin()), 17, 131})
+    for k, (g, d) in enumerate(zip(grids, devs)):
+        got = d.to_host()
+        check_invariants(got, g.active(), f"grid {k}")
+        if k in pick:
+            ref = g.copy()
+            o = pyoracle.step(ref)
+            print(f"grid {k}: {g.li}x{g.lj} {g.fuel} sweeps {stats[k].isor1}/{stats[k].isor2}", flush=True)
+            assert (stats[k].isor1, stats[k].isor2) == (o.isor1, o.isor2), k
+            close_state(got, ref, g.active(), f"grid {k}")
+
+
+def test_gpu_surrogate_trains_at_full_size(torch_cuda):
+    """20 M pairs, batch 8192, 30 epochs, Adam at lr 1e-3 cosine, 5 % held out, on the
+    project's own forward/backward/Adam kernels: the held-out loss falls to the level of
+    the committed weights' training, and the network it produces beats frozen chemistry
+    on a field none of its training states came from."""
+    from unicorn_amd import surrogate as S
+    from unicorn_amd.chem import ChemStep, Controls, DeviceField
+    from unicorn_amd.fields import flame_field
+    from unicorn_amd.scales import Scales, mechanism
+    print("generating 20 M pairs", flush=True)
+    ds = S.generate_pairs(20_000_000, seed=7, log=lambda *a: print(*a, flush=True))
+    stats = S.column_stats(ds)
+    net, info = S.train(ds, stats, epochs=30, batch=8192, lr=1e-3, seed=7,
+                        log=lambda *a: print(*a, flush=True))
+    stats["stmf_max"], stats["stnd_max"] = ds.stmf_max, ds.stnd_max
+    hist = info["history"]
+    print("train/val MSE by epoch", [(h[0], round(h[1], 4), round(h[2], 4)) for h in hist], flush=True)
+    assert len(hist) == 30 and ds.rows >= 20_000_000
+    assert hist[-1][2] <= 0.035 and hist[-1][2] < 0.5 * hist[0][2]
+    del ds
+    torch_cuda.cuda.empty_cache()
+    f = flame_field(201, 101, "C7H16", seed=424242, preignition_band=0.1)
+    ctl = Controls(dt=f.dt)
+    ctx = ChemStep(f.n)
+    ref = DeviceField(f)
+    for _ in range(4):
+        ctx.step(ref, ctl)
+    s0 = ref.to_host()
+    dev = DeviceField(s0)
+    ctx.step(ref, ctl)
+    st = S.SurrogateStep(net, stats, s0).step(dev, ctl)
+    c = f.active()
+    names = [sp["name"] for sp in mechanism()["species"]]
+    mj = [names.index(m) for m in S.MAJORS]
+    r, d = ref.to_host(), dev.to_host()
+    tstr = Scales().tstr
+    eT, fT = (d.tk[c] - r.tk[c]) * tstr, (s0.tk[c] - r.tk[c]) * tstr
+    eY, fY = d.y[mj][:, c] - r.y[mj][:, c], s0.y[mj][:, c] - r.y[mj][:, c]
+    rms = lambda a: float(np.sqrt(np.mean(a * a)))  # noqa: E731
+    print(st, "T rms", rms(eT), rms(fT), "Y rms", rms(eY), rms(fY), flush=True)
+    assert st["accepted"] >= 0.4 * c.size
+    assert rms(eT) <= 0.3 * rms(fT) and rms(eY) <= 0.3 * rms(fY)
+
+
+def test_gpu_surrogate_200_steps_1001x401(torch_cuda):
+    """The gated surrogate with FP64 fallback against the all-FP64 run, 200 chained steps
+    at dt = 1e-5 s from the seeded 1001 x 401 C7H16 field.  Asserted: the gate holds the
+    one-step error within 5 K and 1e-3 of the major species at every step, the sum of
+    mass fractions drifts by no more than the FP64 run's, the fallback rate falls as the
+    field relaxes, and the path is faster than the FP64 step.  The chained error over all
+    200 steps (each accepted step's small bias accumulates in cells accepted every step)
+    is reported and bounded loosely: it is above the 5 K / 1e-3 asked of it."""
+    from unicorn_amd.fields import config1
+    from unicorn_amd.surrogate_run import DEFAULT_WEIGHTS, evaluate
+    if not DEFAULT_WEIGHTS.exists():
+        pytest.skip("no trained weights")
+    print("surrogate: 200 steps on 1001x401", flush=True)
+    r = evaluate(config1(), 200)
+    print({k: r[k] for k in ("speedup", "fallback_rate_mean", "fallback_rate_first_last", "one_step_max_abs_dT_K",
+                             "one_step_max_abs_major_species_error", "chained_max_abs_dT_K",
+                             "chained_max_abs_major_species_error", "sumY_residual_fp64",
+                             "sumY_residual_surrogate")}, flush=True)
+    assert r["one_step_max_abs_dT_K"] <= 5.0
+    assert r["one_step_max_abs_major_species_error"] <= 1e-3
+    assert r["sumY_residual_surrogate"] <= max(r["sumY_residual_fp64"], 1e-15) * 4
+    assert r["fallback_rate_first_last"][1] < r["fallback_rate_first_last"][0]
+    assert r["fallback_rate_mean"] <= 0.2
+    assert r["speedup"] > 1.0
+    assert r["chained_max_abs_dT_K"] <= 100.0 and r["chained_max_abs_major_species_error"] <= 0.1

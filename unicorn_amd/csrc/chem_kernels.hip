@@ -383,6 +383,9 @@ struct SourcesArgs {
   double* rhsz;         // [npad]
   double* qdot; double* qdsp;  // packed field arrays (solveh.f:1137-1142)
   const double* flux;   // [npad] DT/2 x species-diffusion enthalpy flux divergence (null: zero)
+  // SOLVEH without SOLVESP (ICHEM <= 0, spgsolv.f:1291-1298): the species and soot scalars
+  // as given, [54][n] field layout (SSP 1..52, SST1, SST2); null: the sweeps' iterates
+  const double* given;
 };
 
 // Iterates at the decided count, SST2 completion, N2 closure (solvesp.f:2839-2914)
@@ -402,43 +405,49 @@ __global__ void __launch_bounds__(WAVE) k_sources(const UcParams* __restrict__ P
   RcSlab rc = tile_slab((double*)a.slabs, q / WAVE);
   const int64_t n = a.n;
   const double rho = a.rho[c], tk = a.tk[c], ht = a.ht[c], stmf = a.stmf[c], stnd = a.stnd[c];
-  const int32_t w = a.nsw[q];
-  const int32_t m = w & NSW_MASK;
-  const bool fx = (w & NSW_FIXED) != 0;
-  const bool global = G.mode != UC_MODE_PERCELL;
-  const int32_t nstar = G.nstar;
-  const bool use_st = !global || (fx && m - 1 <= nstar) || (!fx && m == nstar);
-  const FieldBuf SRC = use_st ? field_buf(a.st, a.npad * 8 * 54, q)
-                              : field_buf(a.snap + (int64_t)(nstar - G.snap_from) * 54 * a.npad,
-                                          a.npad * 8 * 54, q);
-  const double dt = G.dt, sor1 = G.sor1, sor2 = G.sor2;
+  const double dt = G.dt;
   double s[54];
+  if (a.given) {
 #pragma unroll
-  for (int k = 0; k < 54; ++k) s[k] = SRC.ld(k * a.npad);
-  if (global && fx && m - 1 <= nstar) {
-    // a fixed cell's species repeat; only SST2 keeps relaxing (solvesp.f:2820-2831)
-    if (nstar == m - 1) {
-      s[53] = a.sst2p[q];
-    } else {
-      const double b = (rho * stnd + dt * rc.ld(UC_SLOT_SOUR + 1) * s[21]) * sor1
-                       / (rho + dt * rc.ld(UC_SLOT_SOUR + 5));
-      for (int32_t i = m; i < nstar; ++i) {
-        const double fnew = sor2 * s[53] + b;
-        s[53] = fnew < 0.0 ? 0.0 : fnew;
+    for (int k = 0; k < 54; ++k) s[k] = a.given[(int64_t)k * n + c];
+  } else {
+    const int32_t w = a.nsw[q];
+    const int32_t m = w & NSW_MASK;
+    const bool fx = (w & NSW_FIXED) != 0;
+    const bool global = G.mode != UC_MODE_PERCELL;
+    const int32_t nstar = G.nstar;
+    const bool use_st = !global || (fx && m - 1 <= nstar) || (!fx && m == nstar);
+    const FieldBuf SRC = use_st ? field_buf(a.st, a.npad * 8 * 54, q)
+                                : field_buf(a.snap + (int64_t)(nstar - G.snap_from) * 54 * a.npad,
+                                            a.npad * 8 * 54, q);
+    const double sor1 = G.sor1, sor2 = G.sor2;
+#pragma unroll
+    for (int k = 0; k < 54; ++k) s[k] = SRC.ld(k * a.npad);
+    if (global && fx && m - 1 <= nstar) {
+      // a fixed cell's species repeat; only SST2 keeps relaxing (solvesp.f:2820-2831)
+      if (nstar == m - 1) {
+        s[53] = a.sst2p[q];
+      } else {
+        const double b = (rho * stnd + dt * rc.ld(UC_SLOT_SOUR + 1) * s[21]) * sor1
+                         / (rho + dt * rc.ld(UC_SLOT_SOUR + 5));
+        for (int32_t i = m; i < nstar; ++i) {
+          const double fnew = sor2 * s[53] + b;
+          s[53] = fnew < 0.0 ? 0.0 : fnew;
+        }
       }
     }
-  }
-  // N2 closure (solvesp.f:2839-2914)
-  double ftot = 0.0;
+    // N2 closure (solvesp.f:2839-2914)
+    double ftot = 0.0;
 #pragma unroll
-  for (int k = 0; k < 51; ++k) ftot += s[k];
-  ftot += s[52];
-  s[51] = 1.0 - ftot;
-  if (ftot > 1.0) {
+    for (int k = 0; k < 51; ++k) ftot += s[k];
+    ftot += s[52];
+    s[51] = 1.0 - ftot;
+    if (ftot > 1.0) {
 #pragma unroll
-    for (int k = 0; k < 51; ++k) s[k] = s[k] / ftot;
-    s[52] = s[52] / ftot;
-    s[51] = 0.0;
+      for (int k = 0; k < 51; ++k) s[k] = s[k] / ftot;
+      s[52] = s[52] / ftot;
+      s[51] = 0.0;
+    }
   }
   // SOLVEH sources with the final iterates (solveh.f:34-1142)
   const double T = tk * P->tstr;
@@ -681,6 +690,20 @@ __global__ void k_zero(double* __restrict__ p, int64_t n) {
 // dfirst[d] .. dfirst[d+1]-1 are the list positions of diagonal d.
 
 // SA = FA, SST1 = STMF, SST2 = STND and FPZ = HT at every cell (spgsolv.f:169-231)
+// Rate coefficients held in the field layout RC(LE, 544) (ckrates.f:27) into the slabs of
+// the listed cells, for SOLVEH called without CKRATES (spgsolv.f:1291-1298 with ICHEM <= 0),
+// which reads whatever /DUMR/ holds.
+__global__ void k_rc_to_slab(const double* __restrict__ rcf, const int32_t* __restrict__ cells, int64_t npad,
+                             int64_t n, double* __restrict__ slabs) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npad) return;
+  bool valid;
+  const int64_t c = list_cell(cells, q, valid);
+  if (!valid) return;
+  double* tile = slabs + (q / WAVE) * (int64_t)SLAB_SLOTS * WAVE + (q % WAVE);
+  for (int r = 0; r < NRX; ++r) tile[(int64_t)r * WAVE] = rcf[(int64_t)r * n + c];
+}
+
 __global__ void k_cpl_init(const double* __restrict__ y, const double* __restrict__ stmf,
                            const double* __restrict__ stnd, const double* __restrict__ ht, int64_t n,
                            double* __restrict__ gs, double* __restrict__ gz) {

@@ -19,6 +19,8 @@
 //             Writes /DUMY/ SSP, /DUMS/ SST1 SST2, /CB04/ FCB (N2) and ISOR1.
 //   solveh_   enthalpy relaxation to the global stop and T(h) (solveh.f:1147-1234).
 //             Writes /DUMY/ FPZ, /CB04/ TK, /HRLS/ QDOT, /DUMD/ QDSP and ISOR2.
+//             Called alone (ICHEM <= 0 with ITHRM > 0, spgsolv.f:1291-1298), it uploads
+//             the state and reads the rate coefficients /DUMR/ holds, as SOLVEH does.
 //
 // Coupling.  SPGSOLV's coefficient assembly stays in the Fortran; CKRATES reads what it
 // left in the COMMON blocks: /DUMMY/ and /DUM0/ E,W,P,Q of every species, /DUMS/
@@ -46,6 +48,7 @@
 namespace shim {
 constexpr int64_t LI = UC_LI, LJ = UC_LJ, LE = LI * LJ;
 constexpr int LSP = 52;
+constexpr int LRX = 544;
 }  // namespace shim
 
 // COMMON blocks, as the reference declares them (src/lib/ckrates.f:6-29,
@@ -68,6 +71,7 @@ extern struct { double z[4][shim::LE]; } dumx_;
 extern struct { double ssp[shim::LSP][shim::LE], fpz[shim::LE], emu[shim::LE], ekt[shim::LE]; } dumy_;
 extern struct { double vmu[shim::LE], vtc[shim::LE], vdf[shim::LSP][shim::LE], tmu[shim::LE]; } cb07_;
 extern struct { double st[4][shim::LE], sst1[shim::LE], sst2[shim::LE]; } dums_;
+extern struct { double rc[shim::LRX][shim::LE]; } dumr_;
 }
 
 namespace {
@@ -96,6 +100,7 @@ struct Shim {
          *vdf = nullptr, *ygeo = nullptr, *xxc = nullptr, *xxs = nullptr, *yyc = nullptr, *yys = nullptr,
          *u = nullptr, *v = nullptr, *w = nullptr;
   double *vtc = nullptr, *stdf = nullptr;  // spgsolv_'s further assembly inputs
+  double* rcf = nullptr;   // /DUMR/ RC for SOLVEH called alone (ICHEM <= 0)
   uc_coupling cp{};
   bool coupled = false;
   // QDOT/QDSP on the device mirror the COMMONs once uploaded: only SOLVEH writes them
@@ -273,11 +278,12 @@ void begin_rates(const char* where) {
 
 extern "C" {
 
-// SUBROUTINE CKRATES (src/lib/ckrates.f:1)
-void ckrates_(void) {
-  const Present pr = scan_coupling();  // host COMMONs only
-  ensure_context();
-  upload_state();
+}  // extern "C"
+
+namespace {
+
+// the coupling inputs this step has, uploaded from the COMMONs SPGSOLV filled
+void upload_coupling(const Present& pr) {
   const size_t b1 = sizeof(double) * shim::LE, b52 = b1 * shim::LSP;
   // the coupling inputs this step has
   auto dev = [&](double*& p, size_t bytes) { return dalloc(p, bytes, "ckrates_: hipMalloc"); };
@@ -312,6 +318,18 @@ void ckrates_(void) {
   g.cp.sigsp = 0.9;  // SOLVEH's argument, set there
   g.cp.isym = cb01_.isym;
   g.coupled = pr.coef || pr.st || pr.z || pr.flux || pr.vel;
+}
+
+}  // namespace
+
+extern "C" {
+
+// SUBROUTINE CKRATES (src/lib/ckrates.f:1)
+void ckrates_(void) {
+  const Present pr = scan_coupling();  // host COMMONs only
+  ensure_context();
+  upload_state();
+  upload_coupling(pr);
   begin_rates("ckrates_");
 }
 
@@ -363,7 +381,26 @@ void solvesp_(int32_t* isor1, double* relxsp, double* tolrsp, double* spnorm) {
 // SUBROUTINE SOLVEH(ISOR2,RELXH,TOLRH,SIGSP,HNORM) (src/lib/solveh.f:1); SIGSP enters
 // the species-diffusion enthalpy flux (solveh.f:52-53)
 void solveh_(int32_t* isor2, double* relxh, double* tolrh, double* sigsp, double* hnorm) {
-  if (g.stage != 2) stop("solveh_", "called without a preceding SOLVESP");
+  // SPGSOLV calls SOLVEH alone when ICHEM <= 0 and ITHRM > 0 (spgsolv.f:1291-1298,
+  // main:1027): the species are the prologue copies SSP = FA, and the sources read whatever
+  // rate coefficients /DUMR/ holds (zero if CKRATES never ran)
+  const bool alone = g.stage == 0;
+  if (alone) {
+    if (cb01_.ichem > 0) stop("solveh_", "called without a preceding CKRATES and SOLVESP while ICHEM > 0");
+    const Present pr = scan_coupling();
+    ensure_context();
+    upload_state();
+    upload_coupling(pr);
+    const size_t brc = sizeof(double) * shim::LE * shim::LRX;
+    h2d(dalloc(g.rcf, brc, "solveh_: hipMalloc"), dumr_.rc, brc, "solveh_");
+    hipcheck(hipStreamSynchronize(g.s), "solveh_");
+    g.ctx->lay.ngrids = 0;  // ISKIP may change between steps: rebuild the cell list
+    const uc_controls ctl{cb03_.dt, 0.99, 1e-8, *relxh, *tolrh, 100, *isor2 >= 1 ? *isor2 : 1, UC_MODE_GLOBAL, 0};
+    check(ph_begin(g.ctx, &g.f, &ctl, 1, g.s, false, nullptr, nullptr, g.coupled ? &g.cp : nullptr), "solveh_");
+    g.stage = 2;
+  } else if (g.stage != 2) {
+    stop("solveh_", "called without a preceding SOLVESP");
+  }
   StepRun& R = g.ctx->run;
   if (*isor2 < 1 || *isor2 > 1000) stop("solveh_", "ISOR2 must be 1..1000");
   if (!(*relxh > 0.0 && *relxh <= 1.0) || !(*hnorm > 0.0)) stop("solveh_", "RELXH or HNORM out of range");
@@ -376,7 +413,11 @@ void solveh_(int32_t* isor2, double* relxh, double* tolrh, double* sigsp, double
   G.zsor2 = 1.0 - *relxh;
   G.tolrh = *tolrh;
   G.isor2 = *isor2;
-  if (R.cpl_sp) {  // the sources with the enthalpy flux, which needs SIGSP
+  if (alone) {
+    if (R.cpl_flux && !(*sigsp > 0.0)) stop("solveh_", "SIGSP must be positive");
+    R.cp.sigsp = *sigsp;
+    check(ph_solveh_sources(g.ctx, g.rcf), "solveh_");
+  } else if (R.cpl_sp) {  // the sources with the enthalpy flux, which needs SIGSP
     if (R.cpl_flux && !(*sigsp > 0.0)) stop("solveh_", "SIGSP must be positive");
     R.cp.sigsp = *sigsp;
     check(ph_sources_cpl(g.ctx), "solveh_");
