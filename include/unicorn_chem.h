@@ -231,6 +231,18 @@ int uc_set_tuning(uc_context *ctx, const uc_tuning *tuning);
 int uc_transport_props(uc_context *ctx, const uc_field *field, double *vmu, double *vtc, double *vdf,
                        double *stdf, void *stream);
 
+/* uc_transport_props without a context and without allocating device memory.  No
+ * counterpart in the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The kernel's
+ * table block (about 152 KB: pair tables, fits, constants) lives in one fixed device
+ * array per GPU, kept until the process ends; its address is looked up at the first
+ * call on each device, and the tables for `scales` are uploaded there when they differ
+ * from the last call's (that call synchronises `stream`; later ones only launch).  Limit:
+ * LI*LJ <= 4,000,000 cells.  Calls that overlap on one GPU with different scales race
+ * on the tables.  Same outputs as uc_transport_props, bit for bit. */
+int uc_transport_props_noalloc(const uc_scales *scales, const uc_field *field, double *vmu, double *vtc,
+                               double *vdf, double *stdf, void *stream);
+
 /* The ISKIP contents of the grids changed in place: rebuild the cell list at the
  * next step. */
 int uc_cells_changed(uc_context *ctx);

@@ -70,7 +70,7 @@ class UcTransport(C.Structure):
 EXPORTS = ("uc_abi_version", "uc_last_error", "uc_scales_from_card", "uc_create", "uc_create_batch",
            "uc_destroy", "uc_step", "uc_step_host", "uc_step_batch", "uc_autotune", "uc_set_tuning",
            "uc_get_tuning", "uc_cells_changed", "uc_clip", "uc_set_clip", "uc_step_coupled", "uc_assemble",
-           "uc_transport_props")
+           "uc_transport_props", "uc_transport_props_noalloc")
 
 _lib = None
 
@@ -116,6 +116,7 @@ def library() -> C.CDLL:
     lib.uc_assemble.argtypes = [C.c_void_p, C.POINTER(UcField), C.POINTER(UcControls), C.POINTER(UcTransport),
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.uc_transport_props.argtypes = [C.c_void_p, C.POINTER(UcField)] + [C.c_void_p] * 5
+    lib.uc_transport_props_noalloc.argtypes = [C.POINTER(UcScales), C.POINTER(UcField)] + [C.c_void_p] * 5
     _lib = lib
     return lib
 

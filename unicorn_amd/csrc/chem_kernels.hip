@@ -1425,6 +1425,64 @@ int uc_transport_props(uc_context* ctx, const uc_field* f, double* vmu, double* 
   return UC_OK;
 }
 
+}  // extern "C"
+
+// ---- uc_transport_props without allocation -------------------------------------------
+// The transport kernel's only scratch is its table block (pair tables, fits, constants).
+// For callers that cannot allocate device memory, it lives in this fixed device array,
+// one per GPU for the life of the process; its address is looked up once per device.
+namespace {
+constexpr size_t TR_SCRATCH_DOUBLES = (sizeof(TrTables) + 7) / 8;   // about 152 KB
+constexpr int TR_MAX_DEVICES = 64;
+struct TrScratchState {
+  double* ptr = nullptr;   // device address of this GPU's copy of `scratch`
+  bool loaded = false;     // tables for `key` are in it
+  uc_scales key{};
+};
+TrScratchState tr_state[TR_MAX_DEVICES];
+std::mutex tr_mutex;
+TrTables tr_host;          // host staging of the tables (static storage)
+}  // namespace
+
+static __device__ __align__(256) double scratch[TR_SCRATCH_DOUBLES];
+
+extern "C" {
+
+int uc_transport_props_noalloc(const uc_scales* scales, const uc_field* f, double* vmu, double* vtc, double* vdf,
+                               double* stdf, void* stream) {
+  if (!scales || !f) return fail(UC_E_ARG, "uc_transport_props_noalloc: null argument");
+  if (f->li < 3 || f->lj < 3) return fail(UC_E_ARG, "uc_transport_props_noalloc: grid must be at least 3 x 3");
+  const int64_t n = (int64_t)f->li * f->lj;
+  if (n > UC_MAX_CELLS) return fail(UC_E_ARG, "uc_transport_props_noalloc: more than 4,000,000 cells");
+  if (!f->y || !f->tk) return fail(UC_E_ARG, "uc_transport_props_noalloc: null field array (y and tk are read)");
+  if (!vmu || !vtc || !vdf || !stdf) return fail(UC_E_ARG, "uc_transport_props_noalloc: null output array");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= TR_MAX_DEVICES) return fail(UC_E_UNSUPPORTED, "uc_transport_props_noalloc: device index");
+  hipStream_t s = (hipStream_t)stream;
+  TrScratchState& st = tr_state[dev];
+  {
+    std::lock_guard<std::mutex> lock(tr_mutex);
+    if (!st.ptr) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        return fail(UC_E_HIP, std::string("uc_transport_props_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+      st.ptr = (double*)p;
+    }
+    if (!st.loaded || std::memcmp(&st.key, scales, sizeof(uc_scales)) != 0) {
+      fill_transport_tables(*scales, tr_host);
+      HIPCHK(hipMemcpyAsync(st.ptr, &tr_host, sizeof(TrTables), hipMemcpyHostToDevice, s));
+      HIPCHK(hipStreamSynchronize(s));   // tr_host is reused by the next upload
+      st.key = *scales;
+      st.loaded = true;
+    }
+  }
+  k_transport<<<blocks_for(n, 64), 64, 0, s>>>((const TrTables*)st.ptr, f->y, f->tk, n, vmu, vtc, vdf, stdf);
+  HIPCHK(hipGetLastError());
+  return UC_OK;
+}
+
 int uc_step_coupled(uc_context* ctx, const uc_field* f, const uc_controls* ctl, const uc_coupling* cp, void* stream,
                     uc_stats* st) {
   return step_cpl_impl(ctx, f, ctl, cp, (hipStream_t)stream, st);
