@@ -243,6 +243,35 @@ def assemble_bench(ctx, dev, ctl, f0, reps=20):
             "note": "SPGSOLV's E/W/P/Q, STE..STS, ZE..ZS and EKT for every cell; equal bit for bit to spgsolv.f"}
 
 
+# transport properties (uc_transport_props): algorithmic HBM bytes per cell, 53 doubles read
+# (Y, T) and 55 written (VMU, VTC, 52 VDF, STDF)
+TR_BYTES_PER_CELL = 8 * (53 + 55)
+
+
+def transport_bench(ctx, dev, f0, reps=20):
+    """Time uc_transport_props on the field with HIP events on its stream; with the
+    reference Fortran's cost of the same block (main:843-1005) on the host for comparison
+    when the oracle's program build is present."""
+    import torch
+    s = torch.cuda.current_stream()
+    ctx.transport_props(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        ctx.transport_props(dev)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    achieved = TR_BYTES_PER_CELL * f0.n / (ms * 1e-3)
+    sub = (f0.y[:51] >= 1e-3).sum(axis=0) + 1
+    return {"kernel": "k_transport (uc_transport_props)", "ms_per_call": ms, "cells_per_s": f0.n / (ms * 1e-3),
+            "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "bytes_per_cell": TR_BYTES_PER_CELL,
+            "submixture_species_mean": float(sub.mean()), "submixture_species_max": int(sub.max()),
+            "note": "main:843-1005 for every cell, bit for bit with the reference program's /CB07/; "
+                    "work per cell grows with the square of the sub-mixture size (species with Y >= 1e-3)"}
+
+
 def coupled_bench(args, f0=None, cpu=True):
     """The field with synthetic transport coefficients through uc_step_coupled."""
     import torch
@@ -294,6 +323,7 @@ def coupled_bench(args, f0=None, cpu=True):
            "sweeps": {"global_species": [s.isor1 for s in stats], "global_enthalpy": [s.isor2 for s in stats],
                       "first_segment": [s.first_segment for s in stats]}}
     out["assemble"] = assemble_bench(ctx, dev, ctl, f0)
+    out["transport"] = transport_bench(ctx, dev, f0)
     if cpu and not args.no_cpu:
         nstar = int(np.median([min(s.isor1, 100) for s in stats]))
         nh = int(np.median([min(s.isor2, 100) for s in stats]))
@@ -419,7 +449,8 @@ def main():
                                          "out every step (first steps, so more sweeps than the 20-step average)"}
     if not args.no_coupled:
         c = coupled_bench(args, f0)
-        out["coupled"] = {k: c[k] for k in ("value", "ms_per_step", "roofline", "wavefront", "sweeps", "assemble")}
+        out["coupled"] = {k: c[k] for k in ("value", "ms_per_step", "roofline", "wavefront", "sweeps", "assemble",
+                                            "transport")}
         if "cpu_baseline" in c:
             out["coupled"]["cpu_baseline"] = c["cpu_baseline"]
             out["coupled"]["gpu_over_cpu"] = c["gpu_over_cpu"]

@@ -259,7 +259,11 @@ int uc_get_tuning(const uc_context *ctx, uc_tuning *tuning);
 
 /* ---- Fortran drop-in (libunicorn_fshim.so) ------------------------------ */
 void ckrates_(void);
+/* ISOR1 at most 127 (the rows of the per-sweep residual partials) */
 void solvesp_(int32_t *isor1, double *relxsp, double *tolrsp, double *spnorm);
+/* ISOR2 at most 1000.  Called without a preceding ckrates_ (ICHEM <= 0 with ITHRM > 0,
+ * spgsolv.f:1291-1298) it reads the species SPGSOLV's prologue left in /DUMY/ SSP and the
+ * rate coefficients /DUMR/ holds, as SOLVEH does. */
 void solveh_(int32_t *isor2, double *relxh, double *tolrh, double *sigsp, double *hnorm);
 /* Replaces SUBROUTINE SPGSOLV(ISOR1,RELXSP,TOLRSP,RESDC,ISOR2,RELXH,TOLRH,RESDH,SIGH,SIGSP)
  * (src/lib/spgsolv.f:6) as a whole, for a Fortran build that drops spgsolv.f instead of its

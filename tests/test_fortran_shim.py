@@ -127,17 +127,20 @@ def test_shim_matches_reference_fixtures(name):
 def test_shim_spgsolv_matches_reference(name):
     """The drop-in's spgsolv_ (coefficient assembly on the device, then the coupled
     chemistry, then SPGSOLV's epilogue) against the flang-built spgsolv.f on the same
-    COMMON blocks (tests/golden/assemble.npz's field and transport inputs): sweep counts
-    exact, RESDC/RESDH and the state to the parity bar.  Both normalise the global stops
+    COMMON blocks (tests/golden/assemble.npz's field and transport inputs), three chained
+    calls: sweep counts exact, RESDC/RESDH, the state and QDOT to the parity bar.  Both normalise the global stops
     by the Fortran build's 711 x 131 grid (spgsolv.f:100-101)."""
     from golden_io import load_transport
     if not (refrun.available("shim") and refrun.available("asm")):
         pytest.skip("reference SPGSOLV and shim drivers not built")
     f0, z = load(name)
     tr = load_transport(z)
-    ref = refrun.spgsolv(f0.copy(), tr, driver="asm")
+    # three chained calls: the drop-in keeps QDOT/QDSP resident after the first upload,
+    # predicts the first sweep segment from the previous call's count and rebuilds its cell
+    # list and coefficient buffers every call
+    ref = refrun.spgsolv(f0.copy(), tr, driver="asm", ncalls=3)
     print("shim spgsolv_", ref["isor1"], ref["isor2"], flush=True)
-    got = refrun.spgsolv(f0.copy(), tr, driver="shim")
+    got = refrun.spgsolv(f0.copy(), tr, driver="shim", ncalls=3)
     inter = f0.interior()
     print("shim", got["isor1"], got["isor2"], "max |dY|", np.abs(got["y"][:, inter] - ref["y"][:, inter]).max(),
           "max |dT|/T", np.abs(got["tk"][inter] - ref["tk"][inter]).max() / np.abs(ref["tk"][inter]).max(),
@@ -149,3 +152,5 @@ def test_shim_spgsolv_matches_reference(name):
                   f0.fuel)
     assert_state_close(g, {k: ref[k] for k in ("y", "tk", "ht", "stmf", "stnd")} | {"rho": f0.rho}, inter,
                        "spgsolv_")
+    sc = np.abs(ref["qdot"][inter]).max()
+    assert np.all(np.abs(got["qdot"][inter] - ref["qdot"][inter]) <= 1e-10 * sc), "spgsolv_ QDOT"

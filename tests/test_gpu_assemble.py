@@ -41,7 +41,9 @@ def test_gpu_assembly_matches_spgsolv_bitwise(torch_cuda, name):
     assert np.array_equal(g["ekt"], z["asm_ekt"])
     for k in ("coef", "stc", "zc"):
         assert np.array_equal(g[k][:, cells], z[f"asm_{k}"][:, cells]), k
-    # cells the assembly does not write keep the zeros they started with
+    # a property of the device layer only: cells the assembly does not write keep the zeros
+    # their buffers started with (the reference leaves face conductances there, which only
+    # cells with ISKIP < 3 would read)
     other = np.setdiff1d(np.arange(f.n), cells)
     assert (g["coef"][:, other] == 0.0).all()
 

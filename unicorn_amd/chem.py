@@ -321,7 +321,8 @@ class ChemStep:
         """SPGSOLV's coefficient assembly on the device (uc_assemble): the coupling inputs of
         step_coupled (coefficients, EKT; TMU, VDF, geometry and velocities from tr), in
         new device arrays.  Cells the assembly does not write (boundary, ISKIP >= 10)
-        hold zero coefficients."""
+        hold zeros here; the reference leaves face conductances there (spgsolv.f:233-348),
+        which no relaxed cell reads."""
         import torch
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream

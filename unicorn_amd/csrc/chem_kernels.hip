@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(WAVE) k_sources(const UcParams* __restrict__ P
   const double hconvr = P->gasc / P->wmstr / P->hstr;
   double tmole = 0.0;
 #pragma unroll
-  for (int k = 0; k < NSP; ++k) tmole += s[k] * P->rwm[k];
+  for (int k = 0; k < NSP; ++k) tmole += s[k] / P->wm[k];
   const double tkdi = 1000.0 / T, tkdis = tkdi * tkdi;
   const double p1 = (double)6.6334f + T * (-3.5686e-03 + T * (1.6682e-08 + T * (2.5611e-10 - 2.6558e-14 * T)));
   const double p2 = 1.8741e+01 - tkdi * (1.2131e+02 - 2.7350e+02 * tkdi + 1.9405e+02 * tkdis
@@ -466,8 +466,8 @@ __global__ void __launch_bounds__(WAVE) k_sources(const UcParams* __restrict__ P
       : 1.009e+01 + T * (-1.183e-02 + T * (4.7753e-06 + T * (-5.87209e-10 - 2.5334e-14 * T)));
   const double p4 = -(double)0.23093f - tkdi * ((double)1.12390f - (double)9.41530f * tkdi
                     + (double)2.99880f * tkdis - (double)0.51382f * tkdis * tkdi + 1.86840e-05 * tkdis * tkdis);
-  const double plankc = (p1 * s[12] * P->rwm[12] + p2 * s[9] * P->rwm[9] + p3 * s[8] * P->rwm[8]
-                         + p4 * s[5] * P->rwm[5]) / tmole;
+  const double plankc = (p1 * s[12] / P->wm[12] + p2 * s[9] / P->wm[9] + p3 * s[8] / P->wm[8]
+                         + p4 * s[5] / P->wm[5]) / tmole;
   const double crad = P->alstr / P->rstr / P->vstr / P->vstr / P->vstr;
   const double qrad = crad * (double)0.4f * (4.0 * 5.669e-08 * plankc * (pow(T, 4.0) - 8.1e+09)
                                             + 3.334e-04 * stmf * (rho / 1.9e+03) * T5);
@@ -482,8 +482,8 @@ __global__ void __launch_bounds__(WAVE) k_sources(const UcParams* __restrict__ P
 #pragma unroll
     for (int i = 0; i < 6; ++i) pk[i] = hiT ? Q->nasa[k][7 + i] : Q->nasa[k][i];
     const double hssp = (pk[0] * T + pk[1] * T2 / 2.0 + pk[2] * T3 / 3.0 + pk[3] * T4 / 4.0
-                         + pk[4] * T5 / 5.0) * Q->rwm[k] * hconvr;
-    const double henth = hssp * Q->wmstr * Q->hstr + pk[5] * Q->rwm[k] * Q->gasc;
+                         + pk[4] * T5 / 5.0) / Q->wm[k] * hconvr;
+    const double henth = hssp * Q->wmstr * Q->hstr + pk[5] / Q->wm[k] * Q->gasc;
     const double prd = Q->wm[k] * Q->hfo[k] * pr;
     prod += prd;
     hrr += prd * henth / Q->hfo[k];
@@ -550,6 +550,7 @@ struct FinishArgs {
 };
 
 __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg, FinishArgs a) {
+#pragma clang fp contract(off)   // the reference's operations in its order
   UcP P = (UcP)Pg;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= a.npad) return;
@@ -592,14 +593,14 @@ __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg,
       a1 = a2 = a3 = a4 = a5 = a6 = 0.0;
       asm volatile("" : "+s"(P));
 #pragma unroll
-      for (int k = 0; k < NSP; ++k) {
-        const double sw = s[k] * P->rwm[k];
-        a1 += (hiT ? P->nasa[k][7] : P->nasa[k][0]) * sw;
-        a2 += (hiT ? P->nasa[k][8] : P->nasa[k][1]) * sw / 2.0;
-        a3 += (hiT ? P->nasa[k][9] : P->nasa[k][2]) * sw / 3.0;
-        a4 += (hiT ? P->nasa[k][10] : P->nasa[k][3]) * sw / 4.0;
-        a5 += (hiT ? P->nasa[k][11] : P->nasa[k][4]) * sw / 5.0;
-        a6 += (hiT ? P->nasa[k][12] : P->nasa[k][5]) * sw;
+      for (int k = 0; k < NSP; ++k) {   // A1..A6 in the reference's order (solveh.f:1190-1197)
+        const double wk = P->wm[k];
+        a1 = a1 + (hiT ? P->nasa[k][7] : P->nasa[k][0]) * s[k] / wk;
+        a2 = a2 + (hiT ? P->nasa[k][8] : P->nasa[k][1]) * s[k] / wk / 2.0;
+        a3 = a3 + (hiT ? P->nasa[k][9] : P->nasa[k][2]) * s[k] / wk / 3.0;
+        a4 = a4 + (hiT ? P->nasa[k][10] : P->nasa[k][3]) * s[k] / wk / 4.0;
+        a5 = a5 + (hiT ? P->nasa[k][11] : P->nasa[k][4]) * s[k] / wk / 5.0;
+        a6 = a6 + (hiT ? P->nasa[k][12] : P->nasa[k][5]) * s[k] / wk;
       }
       const double funt = ((a1 + (a2 + (a3 + (a4 + a5 * tpoly) * tpoly) * tpoly) * tpoly) * tpoly + a6)
                           * P->gasc / P->wmstr - enth;
@@ -633,7 +634,7 @@ __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg,
     for (int k = 0; k < NSP; ++k) {
       if (k < 51) rc_ += fabs(s[k] - Y.ld(k * n));
       Y.st(k * n, s[k]);
-      rbar += s[k] * P->rwm[k];
+      rbar = rbar + s[k] / P->wm[k];   // main:1058
     }
     a.ht[c] = fpz;
     a.stmf[c] = sst1;
@@ -654,10 +655,11 @@ done:
 // density of every cell the chemistry did not update (main:1056-1061)
 __global__ void k_density(const UcParams* __restrict__ P, double* __restrict__ rho, const double* __restrict__ y,
                           const double* __restrict__ tk, const uint8_t* __restrict__ done, int64_t n) {
+#pragma clang fp contract(off)   // the reference's operations in its order
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n || done[c]) return;
   double rbar = 0.0;
-  for (int k = 0; k < NSP; ++k) rbar += y[k * n + c] * P->rwm[k];
+  for (int k = 0; k < NSP; ++k) rbar = rbar + y[k * n + c] / P->wm[k];   // main:1058
   rho[c] = P->pback / (P->beta1 * tk[c] * rbar);
 }
 
@@ -863,6 +865,7 @@ __global__ void k_cpl_rdhsp(const UcParams* __restrict__ Pg, const double* __res
                             const double* __restrict__ rho, const double* __restrict__ emu,
                             const double* __restrict__ vdf, const double* __restrict__ tmu,
                             const double* __restrict__ ekt, double sigsp, int64_t n, double* __restrict__ rd) {
+#pragma clang fp contract(off)   // the reference's operations in its order
   UcP P = (UcP)Pg;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
@@ -876,7 +879,7 @@ __global__ void k_cpl_rdhsp(const UcParams* __restrict__ Pg, const double* __res
     double pk[5];
     for (int i = 0; i < 5; ++i) pk[i] = hiT ? P->nasa[k][7 + i] : P->nasa[k][i];
     const double hssp = (pk[0] * T + pk[1] * T2 / 2.0 + pk[2] * T3 / 3.0 + pk[3] * T4 / 4.0 + pk[4] * T5 / 5.0)
-                        * P->rwm[k] * hconvr;
+                        / P->wm[k] * hconvr;   // solveh.f:49-51
     const double v = vdf ? vdf[k * n + c] : 0.0;
     rd[k * n + c] = (e * v + P->rei * tm / sigsp - ek) * hssp;
   }

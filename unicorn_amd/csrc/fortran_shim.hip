@@ -466,8 +466,9 @@ void spgsolv_(int32_t* isor1, double* relxsp, double* tolrsp, double* resdc, int
   h2d(dalloc(g.xxs, bi, W), cb03_.xxs, bi, W);
   h2d(dalloc(g.yyc, bj, W), cb03_.yyc, bj, W);
   h2d(dalloc(g.yys, bj, W), cb03_.yys, bj, W);
-  // the assembly writes the interior cells with ISKIP < 10; elsewhere the coefficients are
-  // zero, as the reference's zero-initialised COMMONs hold them where nothing assembles
+  // the assembly writes the interior cells with ISKIP < 10; elsewhere the device buffers
+  // hold zeros (the reference leaves face conductances there, spgsolv.f:233-348, which only
+  // cells with ISKIP < 3 read, so no result depends on them)
   hipcheck(hipMemsetAsync(dalloc(g.coef, 204 * b1, W), 0, 204 * b1, g.s), W);
   hipcheck(hipMemsetAsync(dalloc(g.stc, 4 * b1, W), 0, 4 * b1, g.s), W);
   hipcheck(hipMemsetAsync(dalloc(g.zc, 4 * b1, W), 0, 4 * b1, g.s), W);

@@ -104,16 +104,16 @@ def emit_rates(t, L):
     L.append(f"  const double lnRUT = log({f32('82.06')} * T);")
     L.append("  UC_SB();")
     # third bodies (ckrates.f:645-731)
-    L.append("  double tbody = 0.0;")
-    for k in range(NSP):
-        L.append(f"  tbody += fa[{k}] * P->rwm[{k}];")
+    L.append("  double tbody = fa[0] / P->wm[0];")
+    for k in range(1, NSP):
+        L.append(f"  tbody = tbody + fa[{k}] / P->wm[{k}];")
     L.append("  tbody_out = tbody;")
     for g in range(1, 11):
         terms = t["tb_groups"][f"TB{g:03d}"]
         expr = "tbody"
         for term in terms:
             c = f32(term["coef"]) if term["coef"] else "1.0"
-            expr += f" {term['sign']} {c} * fa[{term['species'] - 1}] * P->rwm[{term['species'] - 1}]"
+            expr += f" {term['sign']} {c} * fa[{term['species'] - 1}] / P->wm[{term['species'] - 1}]"
         L.append(f"  const double tb{g} = {expr};")
     L.append("  const double tb0 = tbody;")
     L.append("  for (int k = 0; k < 51; ++k) rc.st(UC_SLOT_RHOFA + k, rho * fa[k]);")

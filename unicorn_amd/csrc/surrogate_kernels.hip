@@ -124,6 +124,7 @@ __global__ void k_batch(const float* __restrict__ xraw, const float* __restrict_
 struct Thermo {  // what the gate needs to turn (T, Y) into the field's variables
   double tstr, hstr, gasc, wmstr, pback, beta1;
   double rwm[NSPS], hfo[NSPS];
+  double wm[NSPS];         // /CB02/ WM, for the density in the reference's order
   double nasah[NSPS][14];  // NASA-7 a_i / i (i = 1..5) and a_6, per range: sensible-enthalpy form
   double mel[3][NSPS];   // element (C, H, O) mass per unit mass of each species
   double xlo[NF], xhi[NF];  // training range of the transformed features (with a margin)
@@ -579,10 +580,11 @@ __global__ void __launch_bounds__(64 * NET_WAVES) k_network(InferArgs a) {
 // step in which the gate accepted every cell, so that no FP64 finish ran.
 __global__ void k_density_all(const Thermo* __restrict__ th, const double* __restrict__ y,
                               const double* __restrict__ tk, double* __restrict__ rho, int64_t n) {
+#pragma clang fp contract(off)
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   double rbar = 0.0;
-  for (int k = 0; k < NSPS; ++k) rbar += y[k * n + c] * th->rwm[k];
+  for (int k = 0; k < NSPS; ++k) rbar = rbar + y[k * n + c] / th->wm[k];   // main:1058
   rho[c] = th->pback / (th->beta1 * tk[c] * rbar);
 }
 

@@ -62,7 +62,8 @@ class Norm(C.Structure):
 
 class Thermo(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("tstr", "hstr", "gasc", "wmstr", "pback", "beta1")] + [
-        ("rwm", C.c_double * LSP), ("hfo", C.c_double * LSP), ("nasah", (C.c_double * 14) * LSP),
+        ("rwm", C.c_double * LSP), ("hfo", C.c_double * LSP), ("wm", C.c_double * LSP),
+        ("nasah", (C.c_double * 14) * LSP),
         ("mel", (C.c_double * LSP) * 3), ("xlo", C.c_double * NF),
         ("xhi", C.c_double * NF), ("tlo", C.c_double * NT), ("thi", C.c_double * NT), ("tmin", C.c_double),
         ("tmax", C.c_double), ("stmf_max", C.c_double), ("stnd_max", C.c_double)]
@@ -472,6 +473,7 @@ def thermo_block(stats: dict, sc: Scales | None = None) -> Thermo:
     mel = element_matrix()
     for k in range(LSP):
         th.rwm[k] = 1.0 / sc.wm[k]
+        th.wm[k] = sc.wm[k]
         th.hfo[k] = sc.hfo[k]
         for r in (0, 7):  # per range: a_i / i for i = 1..5, a_6, a_7 (unused)
             for i in range(7):
