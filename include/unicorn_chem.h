@@ -96,6 +96,14 @@ typedef struct {
     int32_t sweep_launches; /* sweep-segment launches this step (coupled step: wavefront steps,
                                one k_sweep_cpl launch each) */
     int32_t first_segment;  /* sweeps in the first segment (the previous count minus one) */
+    int32_t t_out_of_limit; /* cells whose enthalpy lies above the high-range fit at 5000 K
+                               (the condition of solveh.f:1200-1204, message 712, which the
+                               reference never prints: IOVER starts at 1, solveh.f:1170) */
+    int32_t newton_failed;  /* cells whose T(h) Newton iteration did not converge in 100 steps
+                               (solveh.f:1206-1219; the reference prints message 714 for the
+                               first of them in a call and DEBUG(9427) for each) */
+    int32_t first_failed;   /* flat index (I-1)*LJ+(J-1) of the first of those in I-then-J
+                               order, or -1 */
 } uc_stats;
 
 /* Device-resident field (all pointers are device pointers). */
@@ -116,7 +124,7 @@ typedef struct {
                            stiffest first by the previous step's sweeps in this many bins */
     int32_t lpt;        /* 1: tiles of 64 cells run stiffest first (longest-processing-time) */
     int32_t persistent; /* 1: the sweep kernel runs persistent waves over the tile queue */
-    int32_t block;      /* block size of the one-thread-per-cell kernels: 64, 128 or 256 */
+    int32_t block;      /* block size of the one-thread-per-cell kernels: 64, 128, 256 or 512 */
     int32_t graph;      /* 1: each sweep segment is replayed from a captured hipGraph */
 } uc_tuning;
 

@@ -47,6 +47,11 @@ typedef struct {
     double resdc, resdh;
     int64_t cell_sweeps;         /* sum over cells of species sweeps executed */
     int64_t newton_iters;        /* sum over cells of Newton iterations */
+    int64_t t_out_of_limit;      /* cells whose enthalpy lies above the high-range fit at TPOL2
+                                    (the condition of solveh.f:1200-1204, message 712) */
+    int64_t newton_failed;       /* cells whose Newton iteration did not converge in 100 steps
+                                    (solveh.f:1217-1219, message 714) */
+    int64_t first_failed;        /* the first such cell in I-then-J order (flat index), or -1 */
 } uco_stats;
 
 /* Inputs beyond the chemistry state (any pointer may be NULL: zero).  n = li*lj, field
@@ -349,8 +354,12 @@ static void sh_rates(const uc_scales *S, const double *rc, const double *ssp, co
     *qdot = -hrr * rconv;
 }
 
-/* T from h by Newton (solveh.f:1170-1230).  uu = U*U+V*V+W*W.  Returns Newton iterations. */
-static int sh_temperature(const uc_scales *S, double fpz, double uu, const double *ssp, double *tk) {
+/* T from h by Newton (solveh.f:1170-1230).  uu = U*U+V*V+W*W.  Returns Newton iterations;
+ * *flags (may be NULL) gets TH_OUT_OF_LIMIT and TH_NO_ROOT as the reference's conditions for
+ * its messages 712 and 714 (solveh.f:1200-1204, :1217-1219) hold. */
+enum { TH_OUT_OF_LIMIT = 1, TH_NO_ROOT = 2 };
+static int sh_temperature(const uc_scales *S, double fpz, double uu, const double *ssp, double *tk, int *flags) {
+    int fl = 0;
     double enth = (fpz - 0.5 * uu) * S->hstr;
     for (int k = 0; k < NSP; ++k) enth = enth + ssp[k] * S->hfo[k] * S->hstr;
     int ipl = 0;
@@ -372,7 +381,7 @@ static int sh_temperature(const uc_scales *S, double fpz, double uu, const doubl
         double funt = ((a1 + (a2 + (a3 + (a4 + a5 * tpoly) * tpoly) * tpoly) * tpoly) * tpoly + a6)
                       * S->gasc / S->wmstr - enth;
         if (funt >= 1.0e-07) break;
-        if (ipl >= 7) break;
+        if (ipl >= 7) { fl |= TH_OUT_OF_LIMIT; break; }
         ipl = 7;
         if (tkd <= tpoly) tkd = MT_TPOL2 / 2.0;
         tpoly = MT_TPOL2;
@@ -385,6 +394,8 @@ static int sh_temperature(const uc_scales *S, double fpz, double uu, const doubl
         if (fabs(dtkd) <= F32(0.001)) break;
         tkd = tkd - dtkd;
     }
+    if (it > 100) fl |= TH_NO_ROOT;
+    if (flags) *flags = fl;
     if (tkd > MT_TPOL2) tkd = MT_TPOL2;
     const double tkdmin = S->tstr - 1.0;
     if (tkd < tkdmin) tkd = tkdmin;
@@ -423,8 +434,23 @@ double uco_temperature(const double *ref, double fpz, const double *y, double tk
     double sq[54];
     for (int k = 0; k < NSP; ++k) sq[k] = y[k];
     sq[52] = sq[53] = 0.0;
-    sh_temperature(&S, fpz, 0.0, sq, &tk);
+    sh_temperature(&S, fpz, 0.0, sq, &tk, NULL);
     return tk;
+}
+
+/* uco_temperature with the reference's failure conditions: returns TH_OUT_OF_LIMIT (1) |
+ * TH_NO_ROOT (2) and the new tk in *tk_out. */
+int uco_temperature_status(const double *ref, double fpz, const double *y, double tk, double *tk_out) {
+    uc_scales S;
+    uc_scales_init(&S, ref[0], ref[1], ref[2], ref[3], ref[4], ref[5], 1);
+    uc_scales_hfo(&S, MT_NASA);
+    double sq[54];
+    for (int k = 0; k < NSP; ++k) sq[k] = y[k];
+    sq[52] = sq[53] = 0.0;
+    int fl = 0;
+    sh_temperature(&S, fpz, 0.0, sq, &tk, &fl);
+    *tk_out = tk;
+    return fl;
 }
 
 /* Neighbour values of one cell for sp_cell_sweep/sp_cell_soot (nb[9*53]) from the field-layout
@@ -495,7 +521,7 @@ void uco_density(const double *ref, int li, int lj, const double *y, const doubl
 static int step_impl(const uco_params *p, int li, int lj, double *rho, double *y, double *tk, double *ht,
                      double *stmf, double *stnd, const int32_t *iskip, double *qdot, double *qdsp,
                      const uco_coupling *cp, uco_stats *st, double *rc_dump, const int64_t *dump_cells,
-                     int ndump, double *rsd_hist, int32_t *cell_sweeps) {
+                     int ndump, double *rsd_hist, int32_t *cell_sweeps, uint8_t *thflag) {
     uc_scales S;
     uc_scales_init(&S, p->ref[0], p->ref[1], p->ref[2], p->ref[3], p->ref[4], p->ref[5], 1);
     uc_scales_hfo(&S, MT_NASA);
@@ -700,7 +726,9 @@ static int step_impl(const uco_params *p, int li, int lj, double *rho, double *y
         }
     }
     st->isor2 = zconv ? isorz : p->isor2 + 1;
-#pragma omp parallel for schedule(dynamic, 16) num_threads(p->nthreads > 0 ? p->nthreads : 1) reduction(+ : newton)
+    int64_t n_limit = 0, n_noroot = 0, first_failed = INT64_MAX;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(p->nthreads > 0 ? p->nthreads : 1) \
+    reduction(+ : newton, n_limit, n_noroot) reduction(min : first_failed)
     for (int64_t q = 0; q < nint; ++q) {
         const int64_t c = cells[q];
         double uu = 0.0;
@@ -708,8 +736,18 @@ static int step_impl(const uco_params *p, int li, int lj, double *rho, double *y
             const double u = cp->u ? cp->u[c] : 0.0, v = cp->v ? cp->v[c] : 0.0, w = cp->w ? cp->w[c] : 0.0;
             uu = u * u + v * v + w * w;
         }
-        newton += sh_temperature(&S, fpz[q], uu, &s[q * 54], &tk[c]);
+        int fl = 0;
+        newton += sh_temperature(&S, fpz[q], uu, &s[q * 54], &tk[c], &fl);
+        if (thflag) thflag[c] = (uint8_t)fl;
+        if (fl & TH_OUT_OF_LIMIT) ++n_limit;
+        if (fl & TH_NO_ROOT) {
+            ++n_noroot;
+            if (c < first_failed) first_failed = c;
+        }
     }
+    st->t_out_of_limit = n_limit;
+    st->newton_failed = n_noroot;
+    st->first_failed = n_noroot ? first_failed : -1;
     /* SPGSOLV epilogue :1299-1416 (I outer, J inner) */
     double resdh = 0.0, resdc = 0.0;
     int64_t *pos = malloc(sizeof(int64_t) * (size_t)n);
@@ -747,22 +785,24 @@ static int step_impl(const uco_params *p, int li, int lj, double *rho, double *y
 /* One chemistry step on a field (see oracle/ref_driver.c for the sequence), chemistry-only
  * contract.  rc_dump (optional, ndump x (544+6)) receives RC and SOUR of dump_cells after
  * CKRATES.  rsd_hist (optional, isor1 doubles) receives RSD/SPNORM after each species sweep.
- * cell_sweeps (optional, n ints) receives the species sweeps each cell executed. */
+ * cell_sweeps (optional, n ints) receives the species sweeps each cell executed; thflag
+ * (optional, n bytes) the T(h) conditions of each advanced cell (TH_OUT_OF_LIMIT, TH_NO_ROOT). */
 int uco_step(const uco_params *p, int li, int lj, double *rho, double *y, double *tk, double *ht,
              double *stmf, double *stnd, const int32_t *iskip, double *qdot, double *qdsp,
              uco_stats *st, double *rc_dump, const int64_t *dump_cells, int ndump,
-             double *rsd_hist, int32_t *cell_sweeps) {
+             double *rsd_hist, int32_t *cell_sweeps, uint8_t *thflag) {
     return step_impl(p, li, lj, rho, y, tk, ht, stmf, stnd, iskip, qdot, qdsp, NULL, st, rc_dump, dump_cells,
-                     ndump, rsd_hist, cell_sweeps);
+                     ndump, rsd_hist, cell_sweeps, thflag);
 }
 
 /* The same with the coupling inputs cp (uco_coupling; NULL: uco_step).  Returns -1 for the
  * per-cell mode with coefficients, -2 for a flux without geometry. */
 int uco_step_cpl(const uco_params *p, int li, int lj, double *rho, double *y, double *tk, double *ht,
                  double *stmf, double *stnd, const int32_t *iskip, double *qdot, double *qdsp,
-                 const uco_coupling *cp, uco_stats *st, double *rsd_hist, int32_t *cell_sweeps) {
+                 const uco_coupling *cp, uco_stats *st, double *rsd_hist, int32_t *cell_sweeps,
+                 uint8_t *thflag) {
     return step_impl(p, li, lj, rho, y, tk, ht, stmf, stnd, iskip, qdot, qdsp, cp, st, NULL, NULL, 0, rsd_hist,
-                     cell_sweeps);
+                     cell_sweeps, thflag);
 }
 
 /* ------------------------------------------------------------------------------------

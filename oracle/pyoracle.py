@@ -27,7 +27,8 @@ class Params(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("isor1", C.c_int32), ("isor2", C.c_int32), ("resdc", C.c_double),
-                ("resdh", C.c_double), ("cell_sweeps", C.c_int64), ("newton_iters", C.c_int64)]
+                ("resdh", C.c_double), ("cell_sweeps", C.c_int64), ("newton_iters", C.c_int64),
+                ("t_out_of_limit", C.c_int64), ("newton_failed", C.c_int64), ("first_failed", C.c_int64)]
 
 
 class Coupling(C.Structure):
@@ -52,6 +53,9 @@ def lib():
         _lib.uco_scales.restype = None
         _lib.uco_temperature.restype = C.c_double
         _lib.uco_temperature.argtypes = [C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double), C.c_double]
+        _lib.uco_temperature_status.restype = C.c_int
+        _lib.uco_temperature_status.argtypes = [C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double),
+                                                C.c_double, C.POINTER(C.c_double)]
     return _lib
 
 
@@ -72,6 +76,10 @@ class StepOut:
     rc_dump: np.ndarray | None
     rsd_hist: np.ndarray
     per_cell_sweeps: np.ndarray
+    t_out_of_limit: int = 0     # cells with the condition of solveh.f:1200-1204 (message 712)
+    newton_failed: int = 0      # cells whose Newton did not converge (solveh.f:1217-1219, message 714)
+    first_failed: int = -1      # the first of those in I-then-J order (flat index), or -1
+    thflag: np.ndarray | None = None   # (n,) per cell: 1 out of limit, 2 no root
 
 
 def default_ref():
@@ -115,13 +123,15 @@ def step(field, isor1=100, isor2=100, relxsp=0.99, tolrsp=1e-8, relxh=0.99, tolr
     rc_dump = np.zeros((len(dump), 550)) if len(dump) else None
     hist = np.full(max(isor1, 1), np.nan)
     pcs = np.zeros(n, dtype=np.int32)
+    thf = np.zeros(n, dtype=np.uint8)
     if coupling is not None:
         assert not len(dump), "RC dumps: chemistry-only steps"
         keep = []
         cs = _coupling(coupling, keep)
         rc = lib().uco_step_cpl(C.byref(p), field.li, field.lj, ptr(field.rho), ptr(field.y), ptr(field.tk),
                                 ptr(field.ht), ptr(field.stmf), ptr(field.stnd), ptr(iskip, C.c_int32),
-                                ptr(qdot), ptr(qdsp), C.byref(cs), C.byref(st), ptr(hist), ptr(pcs, C.c_int32))
+                                ptr(qdot), ptr(qdsp), C.byref(cs), C.byref(st), ptr(hist), ptr(pcs, C.c_int32),
+                                ptr(thf, C.c_uint8))
         if rc:
             raise ValueError(f"uco_step_cpl: {rc}")
     else:
@@ -129,9 +139,9 @@ def step(field, isor1=100, isor2=100, relxsp=0.99, tolrsp=1e-8, relxh=0.99, tolr
                        ptr(field.ht), ptr(field.stmf), ptr(field.stnd), ptr(iskip, C.c_int32),
                        ptr(qdot), ptr(qdsp), C.byref(st), ptr(rc_dump),
                        ptr(dump, C.c_int64) if len(dump) else None, len(dump), ptr(hist),
-                       ptr(pcs, C.c_int32))
+                       ptr(pcs, C.c_int32), ptr(thf, C.c_uint8))
     return StepOut(st.isor1, st.isor2, st.resdc, st.resdh, st.cell_sweeps, st.newton_iters,
-                   qdot, qdsp, rc_dump, hist, pcs)
+                   qdot, qdsp, rc_dump, hist, pcs, st.t_out_of_limit, st.newton_failed, st.first_failed, thf)
 
 
 def rates_cell(tk, rho, fa, stmf=0.0, stnd=0.0, ref=None):
@@ -150,6 +160,16 @@ def temperature(fpz, y, tk, ref=None):
     r = np.array(ref or default_ref(), dtype=np.float64)
     yy = np.ascontiguousarray(y, dtype=np.float64)
     return lib().uco_temperature(ptr(r), float(fpz), ptr(yy), float(tk))
+
+
+def temperature_status(fpz, y, tk, ref=None):
+    """(flags, T): flags 1 = enthalpy above the high-range fit at 5000 K (solveh.f:1200-1204),
+    2 = Newton did not converge in 100 iterations (solveh.f:1217-1219)."""
+    r = np.array(ref or default_ref(), dtype=np.float64)
+    yy = np.ascontiguousarray(y, dtype=np.float64)
+    out = C.c_double()
+    fl = lib().uco_temperature_status(ptr(r), float(fpz), ptr(yy), float(tk), C.byref(out))
+    return fl, out.value
 
 
 def scales(ref=None):

@@ -122,6 +122,7 @@ class RefResult:
     qdot: np.ndarray
     qdsp: np.ndarray           # (52, n)
     t_chem: float
+    stdout: str = ""           # what the routines printed (SOLVEH's messages 714 and 9427)
 
 
 def write_input(path, field, nsteps=1, isor1=100, isor2=100, relxsp=0.99, tolrsp=1e-8,
@@ -194,5 +195,7 @@ def run(field, nsteps=1, dump_cells=(), driver="ref", **kw) -> RefResult:
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         write_input(fin, field, nsteps=nsteps, dump_cells=dump_cells, **kw)
-        subprocess.run([str(exe), "run", fin, fout], check=True, capture_output=True, text=True)
-        return read_output(fout, field, nsteps, len(dump_cells))
+        p = subprocess.run([str(exe), "run", fin, fout], check=True, capture_output=True, text=True)
+        r = read_output(fout, field, nsteps, len(dump_cells))
+        r.stdout = p.stdout
+        return r

@@ -19,6 +19,12 @@ Cases:
           asm_* (EKT, EMU, E/W/P/Q, STE..STS, ZE..ZS)
   assemble_conv  the same with seed 21, velocity 5, ISYM 0 (planar): about a third of the
           faces convection-dominated, so the hybrid scheme's other branch is pinned too
+  thfail  31x21 C7H16 field (seed 3) with the enthalpy of nine cells set where SOLVEH's
+          T(h) Newton iteration does not converge (six cells) or the enthalpy lies above the
+          high-range fit at 5000 K (three cells), chosen so that each cell's outcome holds
+          under 1e-13 relative changes of its enthalpy and 1e-15 of T: 1 step; the final
+          field and what the reference printed: the (I, J) of its DEBUG(9427) line per
+          failing cell (th_debug_ij) and its FORMAT 714 lines (th_msg)
   transport  the transport properties PROGRAM UNICORN itself computed (main:829-1005, /CB07/
           at its first CKRATES call, dumped by oracle/_ref/unicorn_spy): 600 cells of the
           sample deck's cold start and 2400 of a restart from a 711x131 C7H16 flame field
@@ -95,6 +101,35 @@ def coupled_field():
     return h
 
 
+# (flat cell, HT) of the thfail case: six cells without a T(h) root in 100 Newton steps
+# (negative HT) and three above the high-range fit at 5000 K, in I order
+THFAIL_HT = [(171, "-0x1.213968e9606cep+7"), (218, "0x1.1dfa862aa2ff3p+16"), (255, "-0x1.c31a3dfcab0a7p+5"),
+             (297, "-0x1.a8c89206bb9a4p+5"), (349, "0x1.08fad761b8bd2p+16"), (381, "-0x1.a05b325b1a4a9p+5"),
+             (465, "-0x1.c31a3dfcab0a7p+5"), (517, "0x1.44bd3c6f2ff8fp+16"), (549, "-0x1.f2a416052beafp+5")]
+
+
+def thfail_field():
+    g = flame_field(31, 21, "C7H16", 3)
+    for c, v in THFAIL_HT:
+        g.ht[c] = float.fromhex(v)
+    return g
+
+
+def thfail_case():
+    """The reference on thfail_field(): its final field and the messages SOLVEH printed."""
+    import re
+    g = thfail_field()
+    r = refrun.run(g.copy())
+    dbg = [tuple(int(x) for x in m.groups()) for m in re.finditer(r"DEBUG\(9427\):\s+(\d+)\s+(\d+)", r.stdout)]
+    msg = [l for l in r.stdout.splitlines() if "UNABLE TO FIND THE ROOT" in l or "OUT-OF-LIMIT" in l]
+    arrays = dict(li=g.li, lj=g.lj, dt=g.dt, fuel=g.fuel, in_rho=g.rho, in_y=g.y, in_tk=g.tk, in_ht=g.ht,
+                  in_stmf=g.stmf, in_stnd=g.stnd, in_iskip=g.iskip, s1_isor1=r.isor1, s1_isor2=r.isor2,
+                  s1_rho=r.rho, s1_y=r.y, s1_tk=r.tk, s1_ht=r.ht, s1_stmf=r.stmf, s1_stnd=r.stnd,
+                  th_debug_ij=np.array(dbg, dtype=np.int32).reshape(-1, 2), th_msg=np.array(msg))
+    np.savez_compressed(OUT / "thfail.npz", **arrays)
+    print("thfail:", len(dbg), "failing cells,", len(msg), "messages", flush=True)
+
+
 def transport_case():
     """Sampled cells of the program's own transport properties (oracle/progrun.py)."""
     import tempfile
@@ -129,6 +164,7 @@ CASE_SPECS = {
     "assemble": lambda: assemble_case("assemble", seed=19, velocity=0.3, isym=1),
     "assemble_conv": lambda: assemble_case("assemble_conv", seed=21, velocity=5.0, isym=0),
     "transport": transport_case,
+    "thfail": thfail_case,
 }
 
 ASM_IN = ("vtc", "tmu", "vdf", "stdf", "u", "v", "ygeo", "xxc", "xxs", "yyc", "yys")

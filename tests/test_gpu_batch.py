@@ -91,7 +91,8 @@ def test_batch_equals_grids_alone_and_matches_oracle(torch_cuda):
 
 @pytest.mark.parametrize("tuning", [dict(bins=1, lpt=0, persistent=0, block=256, graph=1),
                                     dict(bins=16, lpt=1, persistent=1, block=64, graph=1),
-                                    dict(bins=32, lpt=0, persistent=1, block=128, graph=1)])
+                                    dict(bins=32, lpt=0, persistent=1, block=128, graph=1),
+                                    dict(bins=4, lpt=1, persistent=1, block=512, graph=1)])
 def test_tuning_changes_no_result(torch_cuda, tuning):
     """Every step (each replays the captured sweep-segment graph one or more times, with
     the step's other launches and host synchronisations in between) against the same
@@ -116,7 +117,7 @@ def test_autotune_leaves_state_and_picks_a_valid_tuning(torch_cuda):
     before = [d.to_host() for d in devs]
     ctx = ChemStep(sum(f.n for f, _ in made), max_grids=len(made))
     t = ctx.autotune(devs, ctls)
-    assert 1 <= t["bins"] <= 32 and t["block"] in (64, 128, 256) and t["lpt"] in (0, 1) and t["persistent"] in (0, 1)
+    assert 1 <= t["bins"] <= 32 and t["block"] in (64, 128, 256, 512) and t["lpt"] in (0, 1) and t["persistent"] in (0, 1)
     assert t["graph"] in (0, 1)
     assert ctx.tuning() == t
     for b, d in zip(before, devs):

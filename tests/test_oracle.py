@@ -313,3 +313,47 @@ def test_assembled_coefficients_drive_the_coupled_step():
     out = pyoracle.step(g, coupling=cp)
     assert out.isor1 < 100 and out.isor2 < 100
     assert np.isfinite(g.y).all() and np.isfinite(g.tk).all()
+
+
+def test_th_failure_conditions_match_the_reference():
+    """SOLVEH's T(h) failure conditions (solveh.f:1170-1227) on the thfail fixture (nine
+    cells with enthalpies planted where the Newton iteration finds no root in 100 steps or
+    the enthalpy lies above the high-range fit at 5000 K), against the flang-built
+    reference: the final field bit for bit; the cells without a root are exactly those the
+    reference printed a DEBUG(9427) line for; the messages the oracle's flags give under
+    the reference's IOVER rule are its FORMAT 714 lines, character for character; the
+    cells above the fit are those whose reference T is the 5000 K clamp with a converged
+    Newton step, and those where the fit evaluated from the reference's final state
+    (numpy, high-range NASA-7 at 5000 K) lies below the enthalpy."""
+    from golden_io import th_messages
+    f, z = load("thfail")
+    g = f.copy()
+    o = pyoracle.step(g)
+    for k in ("rho", "y", "tk", "ht", "stmf", "stnd"):
+        assert np.array_equal(getattr(g, k), z[f"s1_{k}"]), k
+    assert (o.isor1, o.isor2) == (int(z["s1_isor1"][0]), int(z["s1_isor2"][0]))
+    lj = f.lj
+    noroot = sorted(int(c) for c in np.flatnonzero(o.thflag & 2))
+    assert noroot == sorted((i - 1) * lj + (j - 1) for i, j in z["th_debug_ij"])
+    assert o.newton_failed == len(noroot) == 6 and o.first_failed == noroot[0]
+    msgs = th_messages(o.thflag, f.iskip, f.ht, f.li, f.lj)
+    assert msgs == [str(m) for m in z["th_msg"]] and len(msgs) == 4
+    # above the fit: the 5000 K clamp where Newton converged, and the fit below the enthalpy
+    limit = np.flatnonzero(o.thflag & 1)
+    assert o.t_out_of_limit == limit.size == 3
+    sc = pyoracle.scales()
+    tstr, hstr, wmstr, gasc, wm, hfo = sc[4], sc[9], sc[13], sc[14], sc[20:72], sc[72:124]
+    clamp = np.flatnonzero((z["s1_tk"] == 5000.0 / tstr) & ((o.thflag & 2) == 0))
+    assert np.array_equal(clamp, limit)
+    nasa = np.array([[float.fromhex(x) for x in s["nasa7"]] for s in mechanism()["species"]])
+    y, ht = z["s1_y"], z["s1_ht"]
+    inter = f.active()
+    enth = ht[inter] * hstr + (y[:, inter] * (hfo * hstr)[:, None]).sum(axis=0)
+
+    def fit(T, lo):
+        p = nasa[:, 0:6] if lo else nasa[:, 7:13]
+        a = [(p[:, m] / wm / (m + 1 if 0 < m < 5 else 1)) @ y[:, inter] for m in range(6)]
+        return ((a[0] + (a[1] + (a[2] + (a[3] + a[4] * T) * T) * T) * T) * T + a[5]) * gasc / wmstr - enth
+
+    above = inter[(fit(1000.0, True) < 0) & (fit(5000.0, False) < 0)]
+    assert np.array_equal(above, limit)

@@ -42,7 +42,8 @@ class UcStats(C.Structure):
     _fields_ = [("isor1", C.c_int32), ("isor2", C.c_int32), ("resdc", C.c_double),
                 ("resdh", C.c_double), ("cell_sweeps", C.c_int64), ("redo_cells", C.c_int64),
                 ("ms_rates", C.c_double), ("ms_sweeps", C.c_double), ("ms_sources", C.c_double),
-                ("ms_total", C.c_double), ("sweep_launches", C.c_int32), ("first_segment", C.c_int32)]
+                ("ms_total", C.c_double), ("sweep_launches", C.c_int32), ("first_segment", C.c_int32),
+                ("t_out_of_limit", C.c_int32), ("newton_failed", C.c_int32), ("first_failed", C.c_int32)]
 
 
 class UcTuning(C.Structure):
@@ -164,6 +165,9 @@ class StepStats:
     ms_total: float
     sweep_launches: int
     first_segment: int
+    t_out_of_limit: int     # cells above the high-range enthalpy fit at 5000 K (solveh.f:1200-1204)
+    newton_failed: int      # cells whose T(h) Newton did not converge (solveh.f:1206-1219)
+    first_failed: int       # the first of those in I-then-J order (flat index), or -1
 
     @classmethod
     def of(cls, s: UcStats) -> "StepStats":

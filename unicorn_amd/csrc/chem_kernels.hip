@@ -213,7 +213,7 @@ struct RatesArgs {
 
 // rate coefficients, rho*Y_k and soot sources of every cell into its slab
 // (ckrates.f:92-1352)
-__global__ void __launch_bounds__(256) k_rates(const UcParams* __restrict__ Pg, RatesArgs a) {
+__global__ void __launch_bounds__(512) k_rates(const UcParams* __restrict__ Pg, RatesArgs a) {
   UcP P = (UcP)Pg;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= a.npad) return;  // whole waves only: npad is a multiple of 64
@@ -547,9 +547,12 @@ struct FinishArgs {
   double* fpz_out;   // null: SPGSOLV epilogue + density; else FPZ here and TK only
   const double* fpz_in;  // [npad] relaxed FPZ of the coupled relaxation (null: relax here)
   const double* u; const double* v; const double* w;  // [n] velocities of T(h) (null: zero)
+  int32_t* fails;    // [3][max_grids]: cells out of limit, Newton failures, first failed cell
+  uint8_t* thflag;   // [n] per cell: 1 out of limit, 2 no root (written at every listed cell)
+  int32_t max_grids;
 };
 
-__global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg, FinishArgs a) {
+__global__ void __launch_bounds__(512) k_finish(const UcParams* __restrict__ Pg, FinishArgs a) {
 #pragma clang fp contract(off)   // the reference's operations in its order
   UcP P = (UcP)Pg;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -558,6 +561,7 @@ __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg,
   const int64_t c = list_cell(a.cells, q, valid);
   const GridTab& G = a.gt[wave_grid(a.gid, q)];
   double rc_ = 0.0, rh_ = 0.0;
+  bool limit = false, noroot = false;   // the conditions of solveh.f:1200-1204 and :1217-1219
   if (valid) {
     const int64_t n = a.n;
     const FieldBuf SA = field_buf(a.sa, a.npad * 8 * NSP, q), Y = field_buf(a.y, n * 8 * NSP, c);
@@ -604,16 +608,18 @@ __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg,
       }
       const double funt = ((a1 + (a2 + (a3 + (a4 + a5 * tpoly) * tpoly) * tpoly) * tpoly) * tpoly + a6)
                           * P->gasc / P->wmstr - enth;
-      if (funt >= 1.0e-07 || hiT) break;
+      if (funt >= 1.0e-07) break;
+      if (hiT) { limit = true; break; }   // above the high-range fit at TPOL2 (message 712)
       hiT = true;
       if (tkd <= tpoly) tkd = 5000.0 / 2.0;
       tpoly = 5000.0;
     }
     const double eh = enth * P->wmstr / P->gasc;
+    noroot = true;   // until a correction falls within the tolerance (message 714 otherwise)
     for (int itn = 1; itn <= 100; ++itn) {
       const double dtkd = ((a1 + (a2 + (a3 + (a4 + a5 * tkd) * tkd) * tkd) * tkd) * tkd + a6 - eh)
                           / (a1 + (2.0 * a2 + (3.0 * a3 + (4.0 * a4 + 5.0 * a5 * tkd) * tkd) * tkd) * tkd);
-      if (fabs(dtkd) <= (double)0.001f) break;
+      if (fabs(dtkd) <= (double)0.001f) { noroot = false; break; }
       tkd = tkd - dtkd;
     }
     if (tkd > 5000.0) tkd = 5000.0;
@@ -644,6 +650,16 @@ __global__ void __launch_bounds__(256) k_finish(const UcParams* __restrict__ Pg,
     }
   }
 done:
+  if (valid) a.thflag[c] = (uint8_t)((limit ? 1 : 0) | (noroot ? 2 : 0));
+  {
+    const int32_t g = wave_grid(a.gid, q);
+    const unsigned long long bl = __ballot(limit), bn = __ballot(noroot);
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+      if (bl) atomicAdd(&a.fails[g], (int32_t)__popcll(bl));
+      if (bn) atomicAdd(&a.fails[a.max_grids + g], (int32_t)__popcll(bn));
+    }
+    if (noroot) atomicMin(&a.fails[2 * a.max_grids + g], (int32_t)(c - G.off));
+  }
   rc_ = wave_sum(rc_);
   rh_ = wave_sum(rh_);
   if ((threadIdx.x & (WAVE - 1)) == 0) {
@@ -1093,6 +1109,7 @@ struct uc_context {
   int32_t* nsel = nullptr;      // selected-cell count (device) and its pinned host copy
   int32_t* nsel_h = nullptr;
   uint8_t* done = nullptr;      // [max_cells]
+  uint8_t* thflag = nullptr;    // [max_cells] T(h) conditions of the last step (k_finish)
   GridTab* gt_dev = nullptr;    // [max_grids]
   GridTab* gt = nullptr;        // pinned host mirror
   PackPtrs* grid_ptrs = nullptr;  // [max_grids] device copy of the grids' pointers (batches)
@@ -1103,6 +1120,7 @@ struct uc_context {
   double* rows = nullptr;       // [max_grids][128] row sums
   double* rows_h = nullptr;     // pinned host copy
   unsigned long long* counters = nullptr;  // [2][max_grids] sweeps executed, cells re-run
+  int32_t* fails = nullptr;     // [3][max_grids] T(h): cells out of limit, no root, first without root
   hipEvent_t ev[8]{};
   std::vector<int32_t> last_nstar;  // per grid: previous step's species sweep count
   Layout lay;
@@ -1316,6 +1334,7 @@ int uc_create_batch(const uc_scales* scales, int64_t max_cells, int32_t max_grid
   HIPCHK(hipMalloc(&ctx->nsel, sizeof(int32_t) * 64));
   HIPCHK(hipHostMalloc(&ctx->nsel_h, sizeof(int32_t) * 64, hipHostMallocDefault));
   HIPCHK(hipMalloc(&ctx->done, max_cells));
+  HIPCHK(hipMalloc(&ctx->thflag, max_cells));
   HIPCHK(hipMalloc(&ctx->gt_dev, sizeof(GridTab) * max_grids));
   HIPCHK(hipHostMalloc(&ctx->gt, sizeof(GridTab) * max_grids, hipHostMallocDefault));
   HIPCHK(hipMalloc(&ctx->grid_ptrs, sizeof(PackPtrs) * max_grids));
@@ -1324,6 +1343,7 @@ int uc_create_batch(const uc_scales* scales, int64_t max_cells, int32_t max_grid
   HIPCHK(hipMalloc(&ctx->rows, sizeof(double) * 128 * max_grids));
   HIPCHK(hipHostMalloc(&ctx->rows_h, sizeof(double) * 128 * max_grids, hipHostMallocDefault));
   HIPCHK(hipMalloc(&ctx->counters, sizeof(unsigned long long) * 2 * max_grids));
+  HIPCHK(hipMalloc(&ctx->fails, sizeof(int32_t) * 3 * max_grids));
   if (UC_SWEEP_LDS * 512 > 65536) {
     HIPCHK(hipFuncSetAttribute((const void*)k_sweep<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                UC_SWEEP_LDS * 512));
@@ -1353,7 +1373,7 @@ int uc_destroy(uc_context* ctx) {
                   (void*)ctx->keys[1], (void*)ctx->vals[0], (void*)ctx->vals[1], (void*)ctx->order,
                   (void*)ctx->queue, ctx->sort_tmp, (void*)ctx->done, (void*)ctx->gt_dev, (void*)ctx->grid_ptrs,
                   (void*)ctx->rsd_part, (void*)ctx->res_part, (void*)ctx->rows, (void*)ctx->counters,
-                  (void*)ctx->pk.rho, (void*)ctx->pk.y, (void*)ctx->pk.tk, (void*)ctx->pk.ht, (void*)ctx->pk.stmf,
+                  (void*)ctx->fails, (void*)ctx->thflag, (void*)ctx->pk.rho, (void*)ctx->pk.y, (void*)ctx->pk.tk, (void*)ctx->pk.ht, (void*)ctx->pk.stmf,
                   (void*)ctx->pk.stnd, (void*)ctx->pk.qdot, (void*)ctx->pk.qdsp, (void*)ctx->pk_iskip,
                   (void*)ctx->nsel, (void*)ctx->cpl_it, (void*)ctx->cpl_fpz, (void*)ctx->cpl_rd,
                   (void*)ctx->cpl_coef, (void*)ctx->cpl_flux, (void*)ctx->cpl_snap, (void*)ctx->cpl_zsnap,
@@ -1507,8 +1527,8 @@ int uc_autotune(uc_context* ctx, const uc_field* fields, const uc_controls* ctls
 
 int uc_set_tuning(uc_context* ctx, const uc_tuning* t) {
   if (!ctx || !t) return fail(UC_E_ARG, "uc_set_tuning: null argument");
-  if (t->bins < 1 || t->bins > 32 || (t->block != 64 && t->block != 128 && t->block != 256))
-    return fail(UC_E_ARG, "uc_set_tuning: bins must be 1..32 and block 64, 128 or 256");
+  if (t->bins < 1 || t->bins > 32 || (t->block != 64 && t->block != 128 && t->block != 256 && t->block != 512))
+    return fail(UC_E_ARG, "uc_set_tuning: bins must be 1..32 and block 64, 128, 256 or 512");
   ctx->tune = Tuning{t->bins, t->lpt ? 1 : 0, t->persistent ? 1 : 0, t->block, t->graph ? 1 : 0};
   return UC_OK;
 }

@@ -125,6 +125,24 @@ void hipcheck(hipError_t e, const char* where) {
   if (e != hipSuccess) stop(where, hipGetErrorString(e));
 }
 
+// x under the Fortran edit descriptor D12.5: [-]0.ddddd, then D and a signed two-digit
+// exponent (a signed three-digit one without the D beyond 99), right-justified in 12
+std::string fortran_d12_5(double x) {
+  int e = 0;
+  std::string digits = "00000";
+  if (x != 0.0) {
+    char b[32];
+    std::snprintf(b, sizeof b, "%.4e", std::fabs(x));   // d.dddde+XX: the 5 significant digits
+    digits = std::string(1, b[0]) + std::string(b + 2, 4);
+    e = std::atoi(b + 7) + 1;
+  }
+  char ex[8];
+  if (std::abs(e) <= 99) std::snprintf(ex, sizeof ex, "D%c%02d", e < 0 ? '-' : '+', std::abs(e));
+  else std::snprintf(ex, sizeof ex, "%c%03d", e < 0 ? '-' : '+', std::abs(e));
+  std::string s = std::string(std::signbit(x) && x != 0.0 ? "-" : "") + "0." + digits + ex;
+  return s.size() < 12 ? std::string(12 - s.size(), ' ') + s : s;
+}
+
 // /CB06/, /CB02/ WM and PREF, /HFORM/ as the main program set them (main:421-464)
 uc_scales scales_from_commons() {
   uc_scales s{};
@@ -427,6 +445,29 @@ void solveh_(int32_t* isor2, double* relxh, double* tolrh, double* sigsp, double
   uc_stats st{};
   check(ph_end(g.ctx, &st), "solveh_");
   *isor2 = st.isor2;
+  if (st.newton_failed > 0) {
+    // the reference's messages (solveh.f:1170-1227): over the cells in I-then-J order,
+    // IOVER starts at 1, a cell above the high-range fit sets it to 1, and a cell whose
+    // Newton iteration fails prints FORMAT 714 with its HT (which SOLVEH leaves as it was)
+    // while IOVER <= 1, then sets it to 2.  The DEBUG(9427) line the reference prints for
+    // every failing cell is not reproduced (uc_stats counts those cells).
+    std::vector<uint8_t> fl(shim::LE);
+    d2h(fl.data(), g.ctx->thflag, shim::LE, "solveh_");
+    hipcheck(hipStreamSynchronize(g.s), "solveh_");
+    int iover = 1;
+    std::fflush(stdout);
+    for (int64_t i = 1; i < shim::LI - 1; ++i)
+      for (int64_t j = 1; j < shim::LJ - 1; ++j) {
+        const int64_t c = i * shim::LJ + j;
+        if (cb01_.iskip[c] >= 3) continue;
+        if (fl[c] & 1) iover = 1;
+        if (fl[c] & 2) {
+          if (iover <= 1) std::printf("     UNABLE TO FIND THE ROOT%4d  %s\n", (int)(i + 1), fortran_d12_5(cb04_.ht[c]).c_str());
+          iover = 2;
+        }
+      }
+    std::fflush(stdout);
+  }
   const size_t b1 = sizeof(double) * shim::LE;
   d2h(dumy_.fpz, g.fpz, b1, "solveh_");
   d2h(cb04_.tk, g.tk, b1, "solveh_");

@@ -219,12 +219,12 @@ RC_READ = re.compile(r"rc\.ld\(([^()]*)\)")
 
 # register cache and prefetch depth of the sweep's slab reads (doubles)
 SWEEP_CACHE = int(os.environ.get("UC_SWEEP_CACHE", "70"))
-SWEEP_PREFETCH = int(os.environ.get("UC_SWEEP_PREFETCH", "32"))
-SWEEP_LDS = int(os.environ.get("UC_SWEEP_LDS", "126"))     # LDS positions (doubles per lane)
+SWEEP_PREFETCH = int(os.environ.get("UC_SWEEP_PREFETCH", "24"))
+SWEEP_LDS = int(os.environ.get("UC_SWEEP_LDS", "160"))     # LDS positions (doubles per lane)
 # the coupled sweep runs a few waves per wavefront step, so it is bound by load latency,
 # not bandwidth: fewer values cached in registers, more loads in flight
 CPL_CACHE = int(os.environ.get("UC_CPL_CACHE", str(SWEEP_CACHE)))
-CPL_PREFETCH = int(os.environ.get("UC_CPL_PREFETCH", str(SWEEP_PREFETCH)))
+CPL_PREFETCH = int(os.environ.get("UC_CPL_PREFETCH", "32"))
 CPL_BARRIER = int(os.environ.get("UC_CPL_BARRIER", "1"))   # statements between scheduling barriers
 
 

@@ -47,8 +47,9 @@ SPECIES = ("H2", "O2", "H", "O", "OH", "H2O", "HO2", "H2O2", "CO", "CO2", "HCO",
 
 # ----------------------------------------------------------------------------- numbers
 
-def fortran_e(values, width: int = 14, digits: int = 7, scale1p: bool = True, expdigits: int = 0) -> list:
-    """Fortran E editing of each value.
+def fortran_e(values, width: int = 14, digits: int = 7, scale1p: bool = True, expdigits: int = 0,
+              letter: str = "E") -> list:
+    """Fortran E editing of each value (D editing with letter="D").
 
     scale1p=True is ``1PEw.d`` (one digit before the point, d after: 1.2345678E+00);
     False is ``Ew.dEe`` (0.1234568E+001).  expdigits=0 means the default exponent
@@ -73,9 +74,9 @@ def fortran_e(values, width: int = 14, digits: int = 7, scale1p: bool = True, ex
                 e_adj += 1
         if expdigits:
             es = ("+" if e_adj >= 0 else "-") + "%0*d" % (expdigits, abs(e_adj))
-            tail = "E" + es
+            tail = letter + es
         elif abs(e_adj) <= 99:
-            tail = "E" + ("+" if e_adj >= 0 else "-") + "%02d" % abs(e_adj)
+            tail = letter + ("+" if e_adj >= 0 else "-") + "%02d" % abs(e_adj)
         else:
             tail = ("+" if e_adj >= 0 else "-") + "%03d" % abs(e_adj)
         t = mant + tail
