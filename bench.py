@@ -428,6 +428,9 @@ def main():
                 "cell_updates_per_s", "ms_per_step_surrogate", "ms_per_step_fp64", "speedup", "fallback_rate_mean",
                 "one_step_max_abs_dT_K", "one_step_max_abs_major_species_error", "chained_max_abs_dT_K",
                 "chained_max_abs_major_species_error", "sumY_residual_surrogate", "gate", "second_half")}
+            # C, H, O residual of the raw prediction (before the gate's minimum-norm C/H/O
+            # correction), max over the last step's cells; the gate tests the corrected one at tol_el
+            out["surrogate"]["raw_element_residual_max_CHO"] = ev["raw_element_residual_max_last_step"]
             out["surrogate"]["dtype"] = "bf16 network (fp32 accumulation), fp64 gate and fallback"
         else:
             out["surrogate"] = None
