@@ -52,7 +52,11 @@ def expected(z, nsteps: int) -> dict:
 # tolerances of the parity bar (floating point, FP64 throughout)
 REL_Y, ABS_Y = 1e-8, 1e-14      # post-step mass fractions
 REL_T = 1e-8                    # post-step temperature
-REL_RATE = 1e-10                # production rates, relative to the field's largest |rate| of that species
+REL_RATE = 1e-10                # production rates, relative, per value
+# ... plus this fraction of the field's largest |rate| of that species: where a rate is the
+# difference of much larger production and loss sums, the last-bit rounding of those sums
+# (device libm and FMA) is what bounds the agreement; at most one value in 10^4 leans on it
+FLOOR_RATE = 1e-13
 
 
 def assert_state_close(got: ChemField, ref: dict, inter: np.ndarray, label=""):
@@ -70,13 +74,31 @@ def assert_state_close(got: ChemField, ref: dict, inter: np.ndarray, label=""):
         assert np.all(np.abs(a - b) <= REL_Y * np.abs(b) + 1e-300), f"{label}: {name}"
 
 
-def assert_rates_close(qdot, qdsp_cells, ref_qdot, ref_qdsp_cells, inter, label=""):
-    sc = np.abs(ref_qdot[inter]).max()
-    d = np.abs(qdot[inter] - ref_qdot[inter])
-    assert d.max() <= REL_RATE * sc, f"{label}: QDOT max |d| {d.max():.3e} vs scale {sc:.3e}"
+def assert_rates_close(qdot, qdsp_cells, ref_qdot, ref_qdsp_cells, inter, label="", per_value=True):
+    """QDOT at the cells `inter` and QDSP at the cells whose columns are given.
+
+    per_value (one step from identical states): |d| <= REL_RATE |ref| + FLOOR_RATE (largest
+    |ref| of that species), and at most one value in 10^4 beyond REL_RATE relative (each such
+    value is then below 1e-3 of its species' largest rate, by the first bound).  After chained steps the states themselves differ
+    in the last bits, which stiff rates amplify, so there (per_value=False) the bar is
+    REL_RATE of the field's largest |rate| of that species."""
+    a, b = qdot[inter], ref_qdot[inter]
+    sc = np.abs(b).max()
+    d = np.abs(a - b)
+    if not per_value:
+        assert d.max() <= REL_RATE * sc, f"{label}: QDOT max |d| {d.max():.3e} vs scale {sc:.3e}"
+        scale_k = np.maximum(np.abs(ref_qdsp_cells).max(axis=1, keepdims=True), 1e-300)
+        dk = np.abs(qdsp_cells - ref_qdsp_cells)
+        assert np.all(dk <= REL_RATE * scale_k), f"{label}: QDSP max rel-to-scale {(dk / scale_k).max():.3e}"
+        return
+    assert np.all(d <= REL_RATE * np.abs(b) + FLOOR_RATE * sc), \
+        f"{label}: QDOT max rel {(d / np.maximum(np.abs(b), 1e-300)).max():.3e}"
     scale_k = np.maximum(np.abs(ref_qdsp_cells).max(axis=1, keepdims=True), 1e-300)
     dk = np.abs(qdsp_cells - ref_qdsp_cells)
-    assert np.all(dk <= REL_RATE * scale_k), f"{label}: QDSP max rel-to-scale {(dk / scale_k).max():.3e}"
+    ok = dk <= REL_RATE * np.abs(ref_qdsp_cells) + FLOOR_RATE * scale_k
+    assert ok.all(), f"{label}: QDSP {int((~ok).sum())} values off, max rel-to-scale {(dk / scale_k).max():.3e}"
+    over = dk > REL_RATE * np.abs(ref_qdsp_cells)
+    assert over.sum() <= max(2, 1e-4 * over.size), f"{label}: QDSP {int(over.sum())} values beyond 1e-10 relative"
 
 
 def th_messages(thflag, iskip, ht, li, lj):

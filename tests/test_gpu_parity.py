@@ -2,8 +2,11 @@
 
 * against the reference's own outputs (tests/golden): sweep counts exact,
   Y to 1e-8 relative + 1e-14 absolute, T/HT/rho to 1e-8 relative, production
-  rates to 1e-10 of the field's largest rate of that species;
-* against the oracle on larger fields and in the per-cell stop mode;
+  rates after one step per value to 1e-10 relative (plus 1e-13 of the field's largest
+  rate of that species, which only cancellation-dominated values lean on), after chained
+  steps to 1e-10 of that largest rate;
+* against the oracle (bit for bit with the reference on the fixtures) at every active
+  cell of each fixture's field, on larger fields and in the per-cell stop mode;
 * at the full 1001 x 401 C7H16 size: closure, positivity and bounds on every
   cell, and a 2 % random sample of cells re-run by the oracle with the GPU's
   sweep counts.
@@ -50,7 +53,22 @@ def test_gpu_matches_reference_fixtures(torch_cuda, name):
         got = dev.to_host()
         assert_state_close(got, e, inter, f"{name}/{ns}")
         assert_rates_close(dev.qdot.cpu().numpy(), dev.qdsp.cpu().numpy()[:, z["dump_cells"]],
-                           e["qdot"], e["qdsp_dump"], inter, f"{name}/{ns}")
+                           e["qdot"], e["qdsp_dump"], inter, f"{name}/{ns}", per_value=ns == 1)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_gpu_rates_per_value_at_every_cell(torch_cuda, name):
+    """QDOT and QDSP of every active cell after one step from the fixture's input, against
+    the oracle (whose step equals the reference's on these fields bit for bit), per value."""
+    f0, z = load(name)
+    ref = f0.copy()
+    out = pyoracle.step(ref)
+    print(f"case {name}: one step, rates of every cell", flush=True)
+    dev, stats = gpu_run(f0.copy(), 1)
+    assert (stats[0].isor1, stats[0].isor2) == (out.isor1, out.isor2)
+    act = f0.active()
+    assert_rates_close(dev.qdot.cpu().numpy(), dev.qdsp.cpu().numpy()[:, act], out.qdot, out.qdsp[:, act], act,
+                       f"{name}: every cell")
 
 
 @pytest.mark.parametrize("mode", [pyoracle.MODE_GLOBAL, pyoracle.MODE_PERCELL])
@@ -68,7 +86,7 @@ def test_gpu_matches_oracle_c7_band(torch_cuda, mode):
     e = {k: getattr(ref, k) for k in ("rho", "y", "tk", "ht", "stmf", "stnd")}
     assert_state_close(dev.to_host(), e, inter, f"c7 mode {mode}")
     assert_rates_close(dev.qdot.cpu().numpy(), dev.qdsp.cpu().numpy()[:, inter], outs[-1].qdot,
-                       outs[-1].qdsp[:, inter], inter, f"c7 mode {mode}")
+                       outs[-1].qdsp[:, inter], inter, f"c7 mode {mode}", per_value=False)
 
 
 def test_gpu_full_size_properties_and_sample(torch_cuda):
@@ -89,7 +107,7 @@ def test_gpu_full_size_properties_and_sample(torch_cuda):
     sub = f0.take(cells)
     isor1 = min(st.isor1, 100)
     isor2 = min(st.isor2, 100)
-    pyoracle.step(sub, isor1=isor1, tolrsp=-1.0, isor2=isor2, tolrh=-1.0)
+    out = pyoracle.step(sub, isor1=isor1, tolrsp=-1.0, isor2=isor2, tolrh=-1.0)
     si = sub.active()
     assert len(si) == len(cells)
     # sub's interior lists the sample cells in order; got.take lays them out alike
@@ -102,6 +120,9 @@ def test_gpu_full_size_properties_and_sample(torch_cuda):
             assert not bad.any(), (k, bad.sum())
         else:
             assert np.all(np.abs(a[si] - b[si]) <= 1e-8 * np.abs(b[si]) + 1e-300), k
+    # the sample's production rates, per value
+    assert_rates_close(dev.qdot.cpu().numpy()[cells], dev.qdsp.cpu().numpy()[:, cells], out.qdot[si],
+                       out.qdsp[:, si], np.arange(cells.size), "1001x401, 2% sample, one step")
 
 
 def _holed(field, seed, frac=0.3):

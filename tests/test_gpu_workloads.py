@@ -21,7 +21,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from golden_io import ABS_Y, REL_RATE, REL_T, REL_Y
+from golden_io import ABS_Y, REL_T, REL_Y, assert_rates_close
 from oracle import pyoracle
 
 pytestmark = pytest.mark.gpu
@@ -79,10 +79,8 @@ def test_gpu_1001x401_twenty_chained_steps_sample_matches_oracle(torch_cuda):
     # the last step's production rates, per value, at every sampled cell
     qd = dev.qdot.cpu().numpy()[cells]
     qs = dev.qdsp.cpu().numpy()[:, cells]
-    rq, rs = out.qdot[si], out.qdsp[:, si]
-    scale = np.abs(rs).max(axis=1, keepdims=True)
-    assert np.all(np.abs(qd - rq) <= REL_RATE * np.abs(rq).max())
-    assert np.all(np.abs(qs - rs) <= REL_RATE * scale)
+    assert_rates_close(qd, qs, out.qdot[si], out.qdsp[:, si], np.arange(cells.size), "2% sample, step 20",
+                       per_value=False)
 
 
 def test_gpu_256_grids_one_batched_step(torch_cuda):
