@@ -265,6 +265,15 @@ int uc_clip(const uc_field *field, void *stream);
 int uc_set_clip(uc_context *ctx, int32_t on);
 int uc_get_tuning(const uc_context *ctx, uc_tuning *tuning);
 
+/* Per-cell status of the last step's T(h) (the conditions behind SOLVEH's messages,
+ * solveh.f:1200-1227; the reference prints them and continues): flags[li*lj] (device
+ * memory, field layout) of grid `grid` (0 for uc_step) receives 1 where the enthalpy lies
+ * above the high-range fit at 5000 K, 2 where the Newton iteration did not converge in
+ * 100 steps (3: both), 0 elsewhere and at every cell the step did not advance.  The flags
+ * are the context's until its next step; asynchronous on `stream`.  uc_stats carries
+ * their counts. */
+int uc_th_flags(const uc_context *ctx, int32_t grid, uint8_t *flags, void *stream);
+
 /* ---- Fortran drop-in (libunicorn_fshim.so) ------------------------------ */
 void ckrates_(void);
 /* ISOR1 at most 127 (the rows of the per-sweep residual partials) */

@@ -4,7 +4,8 @@ iteration finds no root in 100 steps (six) or the enthalpy lies above the high-r
 at 5000 K (three), each outcome stable under 1e-13 relative changes of the enthalpy.
 
 * uc_step: uc_stats counts both conditions and names the first cell without a root, as
-  the reference's DEBUG(9427) lines and the oracle do; the field to the parity bar.
+  the reference's DEBUG(9427) lines and the oracle do, and uc_th_flags gives the oracle's
+  per-cell flags; the field to the parity bar.
 * uc_step_batch with a second grid: the counts and the first cell stay per grid.
 * The Fortran drop-in (oracle/_ref/ref_driver_shim, libunicorn_fshim.so) prints the
   reference's FORMAT 714 lines, character for character, under its IOVER rule.
@@ -15,7 +16,7 @@ import numpy as np
 import pytest
 
 from golden_io import assert_state_close, load
-from oracle import refrun
+from oracle import pyoracle, refrun
 
 pytestmark = pytest.mark.gpu
 
@@ -38,8 +39,11 @@ def test_gpu_th_failure_counts(torch_cuda):
     f, z = load("thfail")
     print("thfail 31x21: one step on the GPU", flush=True)
     dev = DeviceField(f)
-    st = ChemStep(f.n).step(dev, Controls(dt=f.dt))
+    ctx = ChemStep(f.n)
+    st = ctx.step(dev, Controls(dt=f.dt))
     got = dev.to_host()
+    o = pyoracle.step(f.copy())
+    assert np.array_equal(ctx.th_flags(dev).cpu().numpy(), o.thflag)
     nfail, first = expected(z, f.lj)
     assert (st.isor1, st.isor2) == (int(z["s1_isor1"][0]), int(z["s1_isor2"][0]))
     assert (st.t_out_of_limit, st.newton_failed, st.first_failed) == (3, nfail, first)
@@ -67,6 +71,8 @@ def test_gpu_th_failure_counts_per_grid(torch_cuda):
     nfail, first = expected(z, f.lj)
     assert (st[0].t_out_of_limit, st[0].newton_failed, st[0].first_failed) == (0, 0, -1)
     assert (st[1].t_out_of_limit, st[1].newton_failed, st[1].first_failed) == (3, nfail, first)
+    assert not ctx.th_flags(devs[0], 0).any()
+    assert np.array_equal(ctx.th_flags(devs[1], 1).cpu().numpy(), pyoracle.step(f.copy()).thflag)
 
 
 @pytest.mark.skipif(not refrun.available("shim"), reason="drop-in driver not built (make -C oracle shim)")

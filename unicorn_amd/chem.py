@@ -71,7 +71,7 @@ class UcTransport(C.Structure):
 EXPORTS = ("uc_abi_version", "uc_last_error", "uc_scales_from_card", "uc_create", "uc_create_batch",
            "uc_destroy", "uc_step", "uc_step_host", "uc_step_batch", "uc_autotune", "uc_set_tuning",
            "uc_get_tuning", "uc_cells_changed", "uc_clip", "uc_set_clip", "uc_step_coupled", "uc_assemble",
-           "uc_transport_props", "uc_transport_props_noalloc")
+           "uc_transport_props", "uc_transport_props_noalloc", "uc_th_flags")
 
 _lib = None
 
@@ -104,6 +104,7 @@ def library() -> C.CDLL:
                                 C.POINTER(UcTuning)]
     lib.uc_set_tuning.argtypes = [C.c_void_p, C.POINTER(UcTuning)]
     lib.uc_get_tuning.argtypes = [C.c_void_p, C.POINTER(UcTuning)]
+    lib.uc_th_flags.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
     lib.uc_cells_changed.argtypes = [C.c_void_p]
     lib.uc_clip.argtypes = [C.POINTER(UcField), C.c_void_p]
     lib.uc_set_clip.argtypes = [C.c_void_p, C.c_int32]
@@ -394,6 +395,16 @@ class ChemStep:
     def set_clip(self, on: bool):
         """Whether this context's steps start with main:821-828 (default on)."""
         _check(self.lib.uc_set_clip(self.h, 1 if on else 0))
+
+    def th_flags(self, dev, grid: int = 0):
+        """The last step's per-cell T(h) flags of grid `grid` (dev: its DeviceField), a
+        uint8 device tensor in the field layout: 1 above the high-range fit at 5000 K,
+        2 no Newton root in 100 steps (solveh.f:1200-1227), 0 elsewhere."""
+        import torch
+        out = torch.empty(dev.rho.numel(), dtype=torch.uint8, device=dev.rho.device)
+        _check(self.lib.uc_th_flags(self.h, grid, C.c_void_p(out.data_ptr()),
+                                    C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        return out
 
     def tuning(self) -> dict:
         t = UcTuning()

@@ -1334,7 +1334,7 @@ int uc_create_batch(const uc_scales* scales, int64_t max_cells, int32_t max_grid
   HIPCHK(hipMalloc(&ctx->nsel, sizeof(int32_t) * 64));
   HIPCHK(hipHostMalloc(&ctx->nsel_h, sizeof(int32_t) * 64, hipHostMallocDefault));
   HIPCHK(hipMalloc(&ctx->done, max_cells));
-  HIPCHK(hipMalloc(&ctx->thflag, max_cells));
+  HIPCHK(hipMalloc(&ctx->thflag, (max_cells + 3) / 4 * 4));   // cleared as int32 words
   HIPCHK(hipMalloc(&ctx->gt_dev, sizeof(GridTab) * max_grids));
   HIPCHK(hipHostMalloc(&ctx->gt, sizeof(GridTab) * max_grids, hipHostMallocDefault));
   HIPCHK(hipMalloc(&ctx->grid_ptrs, sizeof(PackPtrs) * max_grids));
@@ -1550,6 +1550,15 @@ int uc_clip(const uc_field* f, void* stream) {
   const int64_t n = (int64_t)f->li * f->lj;
   k_clip<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(f->y, n);
   HIPCHK(hipGetLastError());
+  return UC_OK;
+}
+
+int uc_th_flags(const uc_context* ctx, int32_t grid, uint8_t* flags, void* stream) {
+  if (!ctx || !flags) return fail(UC_E_ARG, "uc_th_flags: null argument");
+  if (grid < 0 || grid >= ctx->run.ngrids) return fail(UC_E_ARG, "uc_th_flags: no such grid in the last step");
+  const GridTab& G = ctx->gt[grid];
+  HIPCHK(hipMemcpyAsync(flags, ctx->thflag + G.off, (size_t)G.li * G.lj, hipMemcpyDeviceToDevice,
+                        (hipStream_t)stream));
   return UC_OK;
 }
 
