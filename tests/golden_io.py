@@ -74,12 +74,14 @@ def assert_state_close(got: ChemField, ref: dict, inter: np.ndarray, label=""):
         assert np.all(np.abs(a - b) <= REL_Y * np.abs(b) + 1e-300), f"{label}: {name}"
 
 
-def assert_rates_close(qdot, qdsp_cells, ref_qdot, ref_qdsp_cells, inter, label="", per_value=True):
+def assert_rates_close(qdot, qdsp_cells, ref_qdot, ref_qdsp_cells, inter, label="", per_value=True,
+                       max_over=1e-4):
     """QDOT at the cells `inter` and QDSP at the cells whose columns are given.
 
     per_value (one step from identical states): |d| <= REL_RATE |ref| + FLOOR_RATE (largest
-    |ref| of that species), and at most one value in 10^4 beyond REL_RATE relative (each such
-    value is then below 1e-3 of its species' largest rate, by the first bound).  After chained steps the states themselves differ
+    |ref| of that species), and at most a fraction max_over of the values beyond REL_RATE
+    relative (each such value is then below 1e-3 of its species' largest rate, by the first
+    bound).  After chained steps the states themselves differ
     in the last bits, which stiff rates amplify, so there (per_value=False) the bar is
     REL_RATE of the field's largest |rate| of that species."""
     a, b = qdot[inter], ref_qdot[inter]
@@ -98,7 +100,8 @@ def assert_rates_close(qdot, qdsp_cells, ref_qdot, ref_qdsp_cells, inter, label=
     ok = dk <= REL_RATE * np.abs(ref_qdsp_cells) + FLOOR_RATE * scale_k
     assert ok.all(), f"{label}: QDSP {int((~ok).sum())} values off, max rel-to-scale {(dk / scale_k).max():.3e}"
     over = dk > REL_RATE * np.abs(ref_qdsp_cells)
-    assert over.sum() <= max(2, 1e-4 * over.size), f"{label}: QDSP {int(over.sum())} values beyond 1e-10 relative"
+    assert over.sum() <= max(2, max_over * over.size), \
+        f"{label}: QDSP {int(over.sum())} of {over.size} values beyond 1e-10 relative"
 
 
 def th_messages(thflag, iskip, ht, li, lj):
