@@ -50,7 +50,7 @@
 extern "C" {
 #endif
 
-#define UC_ABI_VERSION 2
+#define UC_ABI_VERSION 3
 
 enum {
     UC_OK = 0,
@@ -126,6 +126,9 @@ typedef struct {
     int32_t persistent; /* 1: the sweep kernel runs persistent waves over the tile queue */
     int32_t block;      /* block size of the one-thread-per-cell kernels: 64, 128, 256 or 512 */
     int32_t graph;      /* 1: each sweep segment is replayed from a captured hipGraph */
+    int32_t lanes;      /* lanes per cell of the chemistry-only sweep: 1 (thread per cell), or
+                           2 or 4, one lane in each of that many waves of a workgroup, which
+                           split each species block's reaction terms (not with persistent) */
 } uc_tuning;
 
 /* What SPGSOLV hands the chemistry beyond the state (spgsolv.f:98-1290); device pointers

@@ -118,7 +118,8 @@ def test_autotune_leaves_state_and_picks_a_valid_tuning(torch_cuda):
     ctx = ChemStep(sum(f.n for f, _ in made), max_grids=len(made))
     t = ctx.autotune(devs, ctls)
     assert 1 <= t["bins"] <= 32 and t["block"] in (64, 128, 256, 512) and t["lpt"] in (0, 1) and t["persistent"] in (0, 1)
-    assert t["graph"] in (0, 1)
+    assert t["graph"] in (0, 1) and t["lanes"] in (1, 2, 4)
+    assert not (t["lanes"] > 1 and t["persistent"])
     assert ctx.tuning() == t
     for b, d in zip(before, devs):
         assert same(b, d.to_host())

@@ -256,7 +256,13 @@ struct SweepArgs {
 // positions: every cell that has not stopped continues from Y (segment start 0) or
 // from its saved iterates, with the slab's RC.  Sweeps are deterministic functions
 // of the iterates and the slab, so a segment continues the previous one bit for bit.
-__device__ __forceinline__ void sweep_tile(UcP P, const SweepArgs& a, double* lds, int64_t tile) {
+// W lanes per cell: the block's W waves each run one part of the split sweep on the
+// same 64 cells (uc_sweep_mwW) and hold identical iterates; wave 0 writes the results.
+// lds: this lane's first position of its wave's LDS cache; xch: this lane's column of
+// the exchange area (W > 1).
+template <int W>
+__device__ __forceinline__ void sweep_tile(UcP P, const SweepArgs& a, double* lds, double* xch, int64_t tile) {
+  const int part = W == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
   const int64_t q = tile * WAVE + (threadIdx.x & (WAVE - 1));
   const int32_t g = wave_grid(a.gid, q);
   const GridTab& G = a.gt[g];
@@ -274,12 +280,12 @@ __device__ __forceinline__ void sweep_tile(UcP P, const SweepArgs& a, double* ld
     active = valid && (start == 0 || (w0 & NSW_FIXED) == 0);
   }
   const int32_t first = a.redo ? 0 : start;
-  if (a.redo) {
+  if (a.redo && part == 0) {
     const unsigned long long b = __ballot(active);
     if ((threadIdx.x & (WAVE - 1)) == 0 && b) atomicAdd(&a.redo_count[g], (int32_t)__popcll(b));
   }
   if (!__any(active)) {  // residual rows of this tile: zero
-    if (a.rsd_part && (threadIdx.x & (WAVE - 1)) == 0)
+    if (a.rsd_part && threadIdx.x == 0)
       for (int32_t sw = first; sw < first + count; ++sw) a.rsd_part[(int64_t)sw * a.ntiles + tile] = 0.0;
     return;
   }
@@ -313,7 +319,12 @@ __device__ __forceinline__ void sweep_tile(UcP P, const SweepArgs& a, double* ld
       if (active) {
         sst2_prev = s[53];
         double d2;
-        r = uc_sweep(P, rc, lds, s, rho, stmf, stnd, dt, sor1, sor2, &d2);
+        if constexpr (W == 1)
+          r = uc_sweep(P, rc, lds, s, rho, stmf, stnd, dt, sor1, sor2, &d2);
+        else if constexpr (W == 2)
+          r = uc_sweep_mw2(part, P, rc, lds, xch, s, rho, stmf, stnd, dt, sor1, sor2, &d2);
+        else
+          r = uc_sweep_mw4(part, P, rc, lds, xch, s, rho, stmf, stnd, dt, sor1, sor2, &d2);
         ++nsw;
         ++executed;
         if (percell) {
@@ -322,7 +333,7 @@ __device__ __forceinline__ void sweep_tile(UcP P, const SweepArgs& a, double* ld
           active = false;  // every later sweep repeats this one exactly
           fixed = true;
         }
-        if (a.snap && sw + 1 >= snap_from) {
+        if (part == 0 && a.snap && sw + 1 >= snap_from) {
           const FieldBuf SN = field_buf(a.snap + (int64_t)(sw + 1 - snap_from) * 54 * a.npad,
                                         a.npad * 8 * 54, q);
 #pragma unroll
@@ -330,11 +341,12 @@ __device__ __forceinline__ void sweep_tile(UcP P, const SweepArgs& a, double* ld
         }
       }
     }
-    if (a.rsd_part) {
+    if (a.rsd_part && part == 0) {
       const double ws = wave_sum(r);
       if ((threadIdx.x & (WAVE - 1)) == 0) a.rsd_part[(int64_t)sw * a.ntiles + tile] = ws;
     }
   }
+  if (part != 0) return;
   {
     unsigned long long e = executed;
     for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, WAVE);
@@ -359,7 +371,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const UcParams* __restrict__ Pg,
   UcP P = (UcP)Pg;
   if (!PERSISTENT) {
     const int64_t t = blockIdx.x;
-    sweep_tile(P, a, lds, a.order ? a.order[t] : t);
+    sweep_tile<1>(P, a, lds, nullptr, a.order ? a.order[t] : t);
     return;
   }
   for (;;) {
@@ -367,10 +379,25 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const UcParams* __restrict__ Pg,
     if (threadIdx.x == 0) t = atomicAdd(a.queue, 1);
     t = __builtin_amdgcn_readfirstlane(t);  // lane 0's ticket, uniform
     if (t >= a.ntiles) return;  // every wave reaches the end of the queue
-    sweep_tile(P, a, lds, a.order ? __builtin_amdgcn_readfirstlane(a.order[t]) : t);
+    sweep_tile<1>(P, a, lds, nullptr, a.order ? __builtin_amdgcn_readfirstlane(a.order[t]) : t);
     asm volatile("" : "+s"(P));
   }
 }
+
+// W lanes per cell: W waves per block, one tile per block.  Dynamic LDS: each wave's
+// UC_MW<W>_LDS positions per lane, then the exchange area [2][2W+1][64].
+template <int W>
+__global__ void __launch_bounds__(W * WAVE) k_sweep_mw(const UcParams* __restrict__ Pg, SweepArgs a) {
+  extern __shared__ double uc_lds[];
+  constexpr int KL = W == 2 ? UC_MW2_LDS : UC_MW4_LDS;
+  const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+  double* const lds = uc_lds + wv * KL * WAVE + lane;
+  double* const xch = uc_lds + W * KL * WAVE + lane;
+  const int64_t t = blockIdx.x;
+  sweep_tile<W>((UcP)Pg, a, lds, xch, a.order ? a.order[t] : t);
+}
+
+constexpr size_t mw_lds_bytes(int W, int KL) { return (size_t)(W * KL + 2 * (2 * W + 1)) * WAVE * 8; }
 
 struct SourcesArgs {
   const double* rho; const double* tk; const double* ht; const double* stmf; const double* stnd;
@@ -1060,6 +1087,7 @@ struct Tuning {
   int32_t persistent = 1;  // k_sweep as persistent waves over the tile queue
   int32_t block = 256;     // block size of the one-thread-per-cell kernels
   int32_t graph = 1;       // sweep segments replayed from a captured hipGraph
+  int32_t lanes = 1;       // lanes per cell of the chemistry-only sweep (1, 2, 4)
 };
 
 // the sweep segment (grid-table upload, queue reset, k_sweep, per-grid row sums and
@@ -1354,6 +1382,11 @@ int uc_create_batch(const uc_scales* scales, int64_t max_cells, int32_t max_grid
   }
   if (UC_PRD_LDS * 512 > 65536)
     HIPCHK(hipFuncSetAttribute((const void*)k_sources, hipFuncAttributeMaxDynamicSharedMemorySize, UC_PRD_LDS * 512));
+  HIPCHK(hipFuncSetAttribute((const void*)k_sweep_mw<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)mw_lds_bytes(2, UC_MW2_LDS)));
+  HIPCHK(hipFuncSetAttribute((const void*)k_sweep_mw<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)mw_lds_bytes(4, UC_MW4_LDS)));
+
   for (auto& e : ctx->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDefault));
   HIPCHK(hipStreamCreateWithFlags(&ctx->gs, hipStreamNonBlocking));  // default priority
   *out = ctx.release();
@@ -1529,7 +1562,11 @@ int uc_set_tuning(uc_context* ctx, const uc_tuning* t) {
   if (!ctx || !t) return fail(UC_E_ARG, "uc_set_tuning: null argument");
   if (t->bins < 1 || t->bins > 32 || (t->block != 64 && t->block != 128 && t->block != 256 && t->block != 512))
     return fail(UC_E_ARG, "uc_set_tuning: bins must be 1..32 and block 64, 128, 256 or 512");
-  ctx->tune = Tuning{t->bins, t->lpt ? 1 : 0, t->persistent ? 1 : 0, t->block, t->graph ? 1 : 0};
+  if (t->lanes != 1 && t->lanes != 2 && t->lanes != 4)
+    return fail(UC_E_ARG, "uc_set_tuning: lanes must be 1, 2 or 4");
+  if (t->lanes > 1 && t->persistent)
+    return fail(UC_E_ARG, "uc_set_tuning: the split sweep (lanes > 1) runs one tile per workgroup, not persistent");
+  ctx->tune = Tuning{t->bins, t->lpt ? 1 : 0, t->persistent ? 1 : 0, t->block, t->graph ? 1 : 0, t->lanes};
   return UC_OK;
 }
 
@@ -1564,7 +1601,8 @@ int uc_th_flags(const uc_context* ctx, int32_t grid, uint8_t* flags, void* strea
 
 int uc_get_tuning(const uc_context* ctx, uc_tuning* t) {
   if (!ctx || !t) return fail(UC_E_ARG, "uc_get_tuning: null argument");
-  *t = uc_tuning{ctx->tune.bins, ctx->tune.lpt, ctx->tune.persistent, ctx->tune.block, ctx->tune.graph};
+  *t = uc_tuning{ctx->tune.bins, ctx->tune.lpt, ctx->tune.persistent, ctx->tune.block, ctx->tune.graph,
+                 ctx->tune.lanes};
   return UC_OK;
 }
 

@@ -523,21 +523,7 @@ def emit_sweep(t, L, coupled=False):
     missing = stale_expected - stale_found
     extra = stale_found - stale_expected
     assert not missing and not extra, (missing, extra)
-    # soot (solvesp.f:2807-2831)
-    sr = [f"rc.ld(UC_SLOT_SOUR + {q})" for q in range(6)]
-    # soot: PPP - RHS2 with RHS2 = 0 is PPP; with RHS2 = -DT*SOUR6 it is PPP + DT*SOUR6
-    if coupled:
-        num_sm, ppp_sm = neighbour_num("rhs1_soot", 51, 52), "rc.ld(UC_CPL + 255)"
-        num_sn, ppp_sn = neighbour_num("rhs1n", 51, 53), "rc.ld(UC_CPL + 255)"
-    else:
-        num_sm, ppp_sm, num_sn, ppp_sn = "rhs1_soot", "rho", "rhs1n", "rho"
-    S.append(f"const double rhs1_soot = rho * stmf + dt * ({f32('0.012011')} / P->wmstr) * (2.0 * {sr[0]} * s[21]"
-             f" - {sr[2]} * s[1] - {sr[3]} * s[4] - {sr[4]} * s[3]);"
-             f" const double fnew_sm = sor2 * s[52] + ({num_sm}) * sor1 / {ppp_sm};"
-             " rsd += fabs(s[52] - fnew_sm); s[52] = fnew_sm < 0.0 ? 0.0 : fnew_sm;")
-    S.append(f"const double rhs1n = rho * stnd + dt * {sr[1]} * s[21];"
-             f" double fnew_sn = sor2 * s[53] + ({num_sn}) * sor1 / ({ppp_sn} + dt * {sr[5]});"
-             " fnew_sn = fnew_sn < 0.0 ? 0.0 : fnew_sn; *dsst2 = fabs(s[53] - fnew_sn); s[53] = fnew_sn;")
+    S.extend(soot_statements(coupled))   # solvesp.f:2807-2831
     nreads, nlds, nloads = emit_cached(S, L, K, M, SWEEP_LDS, barrier_every=CPL_BARRIER if coupled else 1)
     L.append("  (void)lds;")
     L.append("  return rsd;")
@@ -548,6 +534,201 @@ def emit_sweep(t, L, coupled=False):
 
 
 SO_SLOT = {}
+
+# Split sweep: W lanes per cell, one in each of W waves of a workgroup.  Every species
+# block's reaction terms are divided among the W waves (each RC slot stays with one
+# wave where the balance allows, so a wave reads a quarter of the slab, not all of it);
+# the partial sums meet in LDS, and every wave applies the identical update.
+MW_FORMS = [(2, False), (4, False)]
+MW_LDS = {2: 64, 4: 32}      # LDS positions per lane of each wave
+# register cache and prefetch depth per part
+MW_CACHE = {2: 48, 4: 48}
+MW_PREFETCH = {2: 24, 4: 24}
+SO_READ = re.compile(r"rc\.ld\(UC_SLOT_SO \+ (\d+)\)")
+
+
+def sweep_terms(t):
+    """Per species block (k, A, B, Lo): the reference's production terms (RHS1A, None
+    where the block reuses the previous block's sum), its block-local production terms
+    (RHS1B) and its loss terms (RHS2), as expressions in the reference's order and with
+    the stale product versions of prod_term."""
+    reac = reactants(t)
+    blocks = [b for b in t["sweep"]["blocks"] if b["kind"] != "soot"]
+    doubled = set()
+    for rs in reac.values():
+        for j in set(rs):
+            if rs.count(j) == 2:
+                doubled.add(j)
+    SO_SLOT.clear()
+    SO_SLOT.update({j: i for i, j in enumerate(sorted(doubled))})
+    done, stale, out = set(), set(), []
+    for b in blocks:
+        k = b["k"]
+        A = None
+        if b["rhs1a"] is not None:
+            A = []
+            for term in b["rhs1a"]:
+                if "slot" in term:
+                    A.append(prod_term(term["slot"], reac, done, k, stale, doubled))
+                else:
+                    c = f"{f32(term['coef'])} * " if term.get("coef") else ""
+                    x = f" * s[{term['times'] - 1}]" if term.get("times") else ""
+                    A.append(f"{c}rc.ld(UC_SLOT_SOUR + {term['sour'] - 1}){x}")
+        B = [prod_term(term["slot"], reac, done, k, stale, doubled) for term in b["rhs1b"]]
+        Lo = []
+        for term in b["rhs2"]:
+            if "slot" in term:
+                sl = term["slot"]
+                others = list(reac[sl])
+                others.remove(k)
+                Lo.append("rc.ld(%d)%s" % (sl - 1, "".join(f" * s[{j - 1}]" for j in others)))
+            else:
+                c = f"{f32(term['coef'])} * " if term.get("coef") else ""
+                Lo.append(f"{c}rc.ld(UC_SLOT_SOUR + {term['sour'] - 1})")
+        out.append((k, A, B, Lo))
+        done.add(k)
+    return out
+
+
+def split_terms(blocks, W):
+    """Divide every block's terms among W parts: a term goes to the part that already
+    owns its RC slot unless that part is more than two terms ahead of the least loaded
+    one; otherwise to the least loaded part, which then owns the slot."""
+    owner, parts = {}, []
+    for k, A, B, Lo in blocks:
+        load = [0] * W
+        pb = [([], []) for _ in range(W)]          # (production, loss) per part
+        tagged = [(0, x) for x in (A or []) + B] + [(1, x) for x in Lo]
+        for kind, x in tagged:
+            if x == "0.0":
+                continue
+            sl = [slot_value(v) for v in RC_READ.findall(x)]
+            key = sl[0] if sl else None
+            if key in owner and load[owner[key]] <= min(load) + 2:
+                w = owner[key]
+            else:
+                w = min(range(W), key=lambda i: load[i])
+                owner.setdefault(key, w)
+            load[w] += 1
+            pb[w][kind].append(x)
+        parts.append(pb)
+    return parts
+
+
+def emit_sweep_mw(t, L, W, coupled):
+    """The split sweep (W parts) as W straight-line functions uc_sweep{c}_w{W}_{w} and a
+    dispatcher uc_sweep{c}_mw<W>.  Part w of block k sums its share of the production
+    and loss terms, writes the two partial sums to the exchange area xch (this lane's
+    column of a [2][2W+1][64] LDS array, double-buffered by block), and after an LDS-only
+    barrier reads all W partials and sums them in part order, so every wave forms the
+    same PROD, LOSS and FNEW; part 0 also passes rho*Y_k.  Blocks that reuse the
+    previous block's production sum (no terms) need no exchange."""
+    blocks = sweep_terms(t)
+    parts = split_terms(blocks, W)
+    tag = "_cpl" if coupled else ""
+    K, M, KL = MW_CACHE[W], MW_PREFETCH[W], MW_LDS[W]
+    so = sorted(SO_SLOT.items(), key=lambda x: x[1])
+    stride = 2 * W + 1
+    L.append(f"// split sweep, {W} lanes per cell (one in each of {W} waves){', coupled' if coupled else ''}:")
+    L.append(f"// part w of {W}; same contract as uc_sweep{tag}; xch: this lane's column of the")
+    L.append(f"// workgroup's exchange area, [2][{stride}][64] doubles; lds: this wave's {KL} positions.")
+    L.append(f"#define UC_MW{W}_LDS {KL}")
+    sigs = []
+    for w in range(W):
+        name = f"uc_sweep{tag}_w{W}_{w}"
+        sigs.append(name)
+        L.append("template <class RcPtr>")
+        L.append(f"__device__ __forceinline__ double {name}(UcP P, RcPtr rc, double* __restrict__ lds, double* xch,")
+        L.append("    double (&s)[54], double rho, double stmf, double stnd, double dt, double sor1, double sor2,")
+        L.append("    double* dsst2) {")
+        L.append("  double rsd = 0.0, rhs1a = 0.0;")
+        for j, i in so:
+            L.append(f"  const double so{i} = s[{j - 1}];")
+        L.append("  UC_SB();")
+        S = []
+        xb = 0
+        for bi, ((k, A, B, Lo), pb) in enumerate(zip(blocks, parts)):
+            reused = bi + 1 < len(blocks) and blocks[bi + 1][1] is None
+            prod, loss = pb[w]
+            prod = [SO_READ.sub(lambda m: f"so{m.group(1)}", x) for x in prod]
+            loss = [SO_READ.sub(lambda m: f"so{m.group(1)}", x) for x in loss]
+            if A is None and not B and not Lo:
+                # reuses the previous block's production sum (that block has no RHS1B)
+                S.append(f"const double prod_{k} = rhs1a; const double loss_{k} = 0.0;"
+                         f" const double rhofa_{k} = rc.ld(UC_SLOT_RHOFA + {k - 1});")
+            else:
+                for var, terms in ((f"pp_{k}", prod), (f"pl_{k}", loss)):
+                    if not terms:
+                        S.append(f"const double {var} = 0.0;")
+                    for i in range(0, len(terms), CHUNK):
+                        part = " + ".join(terms[i:i + CHUNK])
+                        S.append(f"double {var} = {part};" if i == 0 else f"{var} = {var} + {part};")
+                base = (xb % 2) * stride
+                xb += 1
+                wr = f"xch[{(base + 2 * w) * 64}] = pp_{k}; xch[{(base + 2 * w + 1) * 64}] = pl_{k};"
+                if w == 0:
+                    wr += f" xch[{(base + 2 * W) * 64}] = rc.ld(UC_SLOT_RHOFA + {k - 1});"
+                S.append(wr + " UC_MW_BARRIER();")
+                ps = " + ".join(f"xch[{(base + 2 * i) * 64}]" for i in range(W))
+                ls = " + ".join(f"xch[{(base + 2 * i + 1) * 64}]" for i in range(W))
+                S.append(f"const double prod_{k} = {ps}; const double loss_{k} = {ls};"
+                         f" const double rhofa_{k} = xch[{(base + 2 * W) * 64}];")
+                if reused:
+                    assert A is not None and not B, "a block whose sum is reused has only RHS1A terms"
+                    S.append(f"rhs1a = prod_{k};")
+            if coupled:
+                upd = (f" const double num_{k} = {neighbour_num(f'rhs1_{k}', k - 1, k - 1)};"
+                       f" const double fnew_{k} = sor2 * s[{k - 1}] + num_{k} * sor1"
+                       f" / (rc.ld(UC_CPL + {5 * (k - 1)}) - rhs2_{k});")
+            else:
+                upd = f" const double fnew_{k} = sor2 * s[{k - 1}] + rhs1_{k} * sor1 / (rho - rhs2_{k});"
+            S.append(f"const double dtw_{k} = dt * P->wm[{k - 1}];"
+                     f" const double rhs1_{k} = rhofa_{k} + dtw_{k} * prod_{k};"
+                     f" const double rhs2_{k} = -dtw_{k} * loss_{k};"
+                     + upd +
+                     f" rsd += fabs(s[{k - 1}] - fnew_{k}); s[{k - 1}] = fnew_{k} < 0.0 ? 0.0 : fnew_{k};"
+                     " asm volatile(\"\" : \"+v\"(rsd));")
+        S.extend(soot_statements(coupled))
+        if xb % 2:
+            # an odd number of exchanges: the next sweep's first one reuses the buffer of this
+            # sweep's last, which a slower wave may still be reading
+            S.append("UC_MW_BARRIER();")
+        nreads, nlds, nloads = emit_cached(S, L, K, M, KL)
+        L.append("  (void)lds;")
+        L.append("  return rsd;")
+        L.append("}")
+        L.append(f"// part {w}: slab reads per sweep: {nreads} written, {nlds} from LDS, {nloads} loaded;"
+                 f" {sum(len(pb[w][0]) + len(pb[w][1]) for pb in parts)} terms")
+    L.append("template <class RcPtr>")
+    L.append(f"__device__ __forceinline__ double uc_sweep{tag}_mw{W}(int part, UcP P, RcPtr rc, double* __restrict__ lds,")
+    L.append("    double* xch, double (&s)[54], double rho, double stmf, double stnd, double dt, double sor1,")
+    L.append("    double sor2, double* dsst2) {")
+    L.append("  switch (part) {")
+    for w, name in enumerate(sigs):
+        L.append(f"    case {w}: return {name}(P, rc, lds, xch, s, rho, stmf, stnd, dt, sor1, sor2, dsst2);")
+    L.append("    default: return 0.0;")
+    L.append("  }")
+    L.append("}")
+    crit = sum(max(len(pb[w][0]) + len(pb[w][1]) for w in range(W)) for pb in parts)
+    L.append(f"// critical path: {crit} terms per sweep (2132 in one lane)")
+    L.append("")
+
+
+def soot_statements(coupled):
+    """The two soot scalars (solvesp.f:2807-2831), as uc_sweep forms them."""
+    sr = [f"rc.ld(UC_SLOT_SOUR + {q})" for q in range(6)]
+    if coupled:
+        num_sm, ppp_sm = neighbour_num("rhs1_soot", 51, 52), "rc.ld(UC_CPL + 255)"
+        num_sn, ppp_sn = neighbour_num("rhs1n", 51, 53), "rc.ld(UC_CPL + 255)"
+    else:
+        num_sm, ppp_sm, num_sn, ppp_sn = "rhs1_soot", "rho", "rhs1n", "rho"
+    return [f"const double rhs1_soot = rho * stmf + dt * ({f32('0.012011')} / P->wmstr) * (2.0 * {sr[0]} * s[21]"
+            f" - {sr[2]} * s[1] - {sr[3]} * s[4] - {sr[4]} * s[3]);"
+            f" const double fnew_sm = sor2 * s[52] + ({num_sm}) * sor1 / {ppp_sm};"
+            " rsd += fabs(s[52] - fnew_sm); s[52] = fnew_sm < 0.0 ? 0.0 : fnew_sm;",
+            f"const double rhs1n = rho * stnd + dt * {sr[1]} * s[21];"
+            f" double fnew_sn = sor2 * s[53] + ({num_sn}) * sor1 / ({ppp_sn} + dt * {sr[5]});"
+            " fnew_sn = fnew_sn < 0.0 ? 0.0 : fnew_sn; *dsst2 = fabs(s[53] - fnew_sn); s[53] = fnew_sn;"]
 
 
 def prod_term(slot, reac, done, k, stale_found, doubled_species):
@@ -663,6 +844,10 @@ def main(out=None):
          "// P is redefined the same way: the parameter block's hundreds of scalars must be",
          "// re-read where used, not hoisted out of loops into registers.",
          "#define UC_SB() do { __builtin_amdgcn_sched_barrier(0); rc.fence(); asm volatile(\"\" : \"+s\"(P)); } while (0)",
+         "// UC_MW_BARRIER(): the split sweep's exchange barrier; orders LDS only, so the",
+         "// slab loads in flight stay in flight across it.",
+         "#define UC_MW_BARRIER() do { __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"workgroup\", \"local\"); \\",
+         "    __builtin_amdgcn_s_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"workgroup\", \"local\"); } while (0)",
          "#define UC_SLOT_RHOFA 544",
          "#define UC_SLOT_G 595",
          "#define UC_SLOT_SOUR 647",
@@ -677,6 +862,8 @@ def main(out=None):
     emit_soot(L)
     emit_sweep(t, L)
     emit_sweep(t, L, coupled=True)
+    for W, coupled in MW_FORMS:
+        emit_sweep_mw(t, L, W, coupled)
     emit_prd(t, L)
     text = "\n".join(L) + "\n"
     dst = Path(out) if out else HERE / "chem_gen.inc"
