@@ -277,6 +277,13 @@ int uc_get_tuning(const uc_context *ctx, uc_tuning *tuning);
  * their counts. */
 int uc_th_flags(const uc_context *ctx, int32_t grid, uint8_t *flags, void *stream);
 
+/* The species sweeps' residual history of the last step of grid `grid`: RSD/SPNORM after
+ * each sweep up to the decided count, the quantity SOLVESP tests against TOLRSP
+ * (solvesp.f:2834) and keeps to itself.  Writes at most `cap` values to `out` (host
+ * memory) and returns how many the step has (a count >= 0), or a negative error code.
+ * Empty for the per-cell stop mode. */
+int uc_sweep_residuals(const uc_context *ctx, int32_t grid, double *out, int32_t cap);
+
 /* ---- Fortran drop-in (libunicorn_fshim.so) ------------------------------ */
 void ckrates_(void);
 /* ISOR1 at most 127 (the rows of the per-sweep residual partials) */

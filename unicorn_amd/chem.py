@@ -71,7 +71,7 @@ class UcTransport(C.Structure):
 EXPORTS = ("uc_abi_version", "uc_last_error", "uc_scales_from_card", "uc_create", "uc_create_batch",
            "uc_destroy", "uc_step", "uc_step_host", "uc_step_batch", "uc_autotune", "uc_set_tuning",
            "uc_get_tuning", "uc_cells_changed", "uc_clip", "uc_set_clip", "uc_step_coupled", "uc_assemble",
-           "uc_transport_props", "uc_transport_props_noalloc", "uc_th_flags")
+           "uc_transport_props", "uc_transport_props_noalloc", "uc_th_flags", "uc_sweep_residuals")
 
 _lib = None
 
@@ -105,6 +105,7 @@ def library() -> C.CDLL:
     lib.uc_set_tuning.argtypes = [C.c_void_p, C.POINTER(UcTuning)]
     lib.uc_get_tuning.argtypes = [C.c_void_p, C.POINTER(UcTuning)]
     lib.uc_th_flags.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+    lib.uc_sweep_residuals.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
     lib.uc_cells_changed.argtypes = [C.c_void_p]
     lib.uc_clip.argtypes = [C.POINTER(UcField), C.c_void_p]
     lib.uc_set_clip.argtypes = [C.c_void_p, C.c_int32]
@@ -404,6 +405,17 @@ class ChemStep:
         out = torch.empty(dev.rho.numel(), dtype=torch.uint8, device=dev.rho.device)
         _check(self.lib.uc_th_flags(self.h, grid, C.c_void_p(out.data_ptr()),
                                     C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        return out
+
+    def sweep_residuals(self, grid: int = 0) -> np.ndarray:
+        """RSD/SPNORM after each species sweep of the last step of grid `grid`, up to the
+        decided count (the quantity solvesp.f:2834 tests against TOLRSP)."""
+        n = self.lib.uc_sweep_residuals(self.h, grid, None, 0)
+        if n < 0:
+            _check(n)
+        out = np.zeros(n)
+        if n:
+            self.lib.uc_sweep_residuals(self.h, grid, out.ctypes.data_as(C.c_void_p), n)
         return out
 
     def tuning(self) -> dict:

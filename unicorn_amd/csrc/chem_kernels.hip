@@ -1064,6 +1064,7 @@ struct StepRun {
   std::vector<uc_controls> ctl;       // their controls
   std::vector<double> spnorm, hnorm;  // global-stop norms per grid
   std::vector<uc_stats> out;          // per-grid statistics
+  std::vector<std::vector<double>> hist;  // per grid: RSD/SPNORM of each species sweep the stop read
 };
 
 // the cell list of the current grids (rebuilt when the grids change)
@@ -1588,6 +1589,15 @@ int uc_clip(const uc_field* f, void* stream) {
   k_clip<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(f->y, n);
   HIPCHK(hipGetLastError());
   return UC_OK;
+}
+
+int uc_sweep_residuals(const uc_context* ctx, int32_t grid, double* out, int32_t cap) {
+  if (!ctx || cap < 0 || (cap > 0 && !out)) return fail(UC_E_ARG, "uc_sweep_residuals: bad argument");
+  if (grid < 0 || grid >= (int32_t)ctx->run.hist.size())
+    return fail(UC_E_ARG, "uc_sweep_residuals: no step of grid " + std::to_string(grid));
+  const std::vector<double>& h = ctx->run.hist[grid];
+  for (int32_t i = 0; i < cap && i < (int32_t)h.size(); ++i) out[i] = h[i];
+  return (int32_t)h.size();
 }
 
 int uc_th_flags(const uc_context* ctx, int32_t grid, uint8_t* flags, void* stream) {
