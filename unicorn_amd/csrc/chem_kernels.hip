@@ -180,14 +180,79 @@ constexpr int32_t NSW_MASK = NSW_FIXED - 1;
 // grid g's cells are field cells off .. off+li*lj-1 of the packed arrays.
 struct GridTab {
   double dt, sor1, sor2, tolrsp, zsor1, zsor2, tolrh;
+  double spnorm, hnorm;                   // global-stop norms (solvesp.f:2834, solveh.f:1162)
   int32_t mode, isor1, isor2;
   int32_t start, count, snap_from, redo;  // current sweep segment (counts are 1-based)
   int32_t nstar, nh;                      // decided species / enthalpy sweep counts
   int32_t hfrom;                          // first enthalpy sweep of the current k_hres chunk
+  int32_t pred, grow, nsnap;              // segment plan: predicted count, next growth, snapshots
+  int32_t isor1_out, isor2_out, hcount;   // reported counts (limit + 1: not converged); residuals kept
   int32_t tile0, tile1;                   // the grid's tiles of 64 list positions
   int32_t li, lj;
   int64_t off;
 };
+
+constexpr int32_t UC_HIST = 128;  // species residuals kept per grid (isor1 <= 127)
+
+// After a sweep segment: each grid's global stop (solvesp.f:2834: the first sweep with
+// RSD/SPNORM <= TOLRSP) from its per-sweep row sums, and its next segment: one sweep,
+// then 2, 4, ... up to the snapshot depth after a predicted first segment, the snapshot
+// depth without a prediction.  A grid that stops keeps the snapshot window of its last
+// segment; when the decided count lies before that window, redo is set to it (the
+// cells not covered re-run from scratch).  hist [grid][UC_HIST] receives RSD/SPNORM of
+// every sweep read.  One thread per grid; the step needs no host round trip per segment.
+__global__ void k_decide_sp(GridTab* __restrict__ gt, const double* __restrict__ rows, int32_t nrows,
+                            int32_t ngrids, double* __restrict__ hist) {
+  for (int32_t g = threadIdx.x; g < ngrids; g += blockDim.x) {
+    GridTab& G = gt[g];
+    if (G.count <= 0) continue;
+    const int32_t end = G.start + G.count;
+    const bool percell = G.mode == UC_MODE_PERCELL;
+    int32_t conv = 0;
+    if (!percell) {
+      const double* r = rows + (int64_t)g * nrows;
+      for (int32_t i = G.start; i < end; ++i) {
+        const double v = r[i - G.start] / G.spnorm;
+        hist[(int64_t)g * UC_HIST + i] = v;
+        if (v <= G.tolrsp) { conv = i + 1; break; }
+      }
+    }
+    if (percell || conv || end >= G.isor1) {
+      G.nstar = conv ? conv : G.isor1;
+      G.isor1_out = conv ? conv : G.isor1 + 1;
+      G.hcount = percell ? 0 : (conv ? conv : end);
+      G.count = 0;
+      G.redo = !percell && G.nstar < G.snap_from ? G.nstar : 0;
+      continue;
+    }
+    const int32_t len = min(G.isor1 - end, G.pred > 0 ? G.grow : G.nsnap);
+    G.grow = min(2 * G.grow, G.nsnap);
+    G.start = end;
+    G.count = len;
+    G.snap_from = end + 1;
+  }
+}
+
+// After a k_hres chunk of H sweeps: each grid's enthalpy stop (solveh.f:1162: the first
+// sweep with RSDH/HNORM <= TOLRH) from its row sums; hfrom advances to the next chunk
+// or becomes -1 with the decided count in nh.
+__global__ void k_decide_h(GridTab* __restrict__ gt, const double* __restrict__ rows, int32_t H, int32_t ngrids) {
+  for (int32_t g = threadIdx.x; g < ngrids; g += blockDim.x) {
+    GridTab& G = gt[g];
+    if (G.hfrom < 0) continue;
+    const int32_t end = min(G.hfrom + H, G.isor2);
+    int32_t conv = 0;
+    for (int32_t i = G.hfrom; i < end; ++i)
+      if (rows[(int64_t)g * H + i - G.hfrom] / G.hnorm <= G.tolrh) { conv = i + 1; break; }
+    if (conv || end >= G.isor2) {
+      G.nh = conv ? conv : G.isor2;
+      G.isor2_out = conv ? conv : G.isor2 + 1;
+      G.hfrom = -1;
+    } else {
+      G.hfrom = end;
+    }
+  }
+}
 
 // The cell list: position q holds a packed field index, or -1-c for a padding lane
 // (c: a real cell of the same grid, so the lane has valid inputs).  Every grid's
@@ -1065,6 +1130,8 @@ struct StepRun {
   std::vector<double> spnorm, hnorm;  // global-stop norms per grid
   std::vector<uc_stats> out;          // per-grid statistics
   std::vector<std::vector<double>> hist;  // per grid: RSD/SPNORM of each species sweep the stop read
+  bool hist_dev = false;              // the history is the device's (k_decide_sp), not `hist`
+  bool span[3] = {false, false, false};  // rates, sweeps, sources timed by ctx->tev pairs
 };
 
 // the cell list of the current grids (rebuilt when the grids change)
@@ -1148,9 +1215,11 @@ struct uc_context {
   double* res_part = nullptr;   // [2][max_tiles]
   double* rows = nullptr;       // [max_grids][128] row sums
   double* rows_h = nullptr;     // pinned host copy
+  double* hist_dev = nullptr;   // [max_grids][UC_HIST] species RSD/SPNORM per sweep (k_decide_sp)
   unsigned long long* counters = nullptr;  // [2][max_grids] sweeps executed, cells re-run
   int32_t* fails = nullptr;     // [3][max_grids] T(h): cells out of limit, no root, first without root
   hipEvent_t ev[8]{};
+  hipEvent_t tev[6]{};  // start/end of the rates, sweeps and sources phases, read at ph_end
   std::vector<int32_t> last_nstar;  // per grid: previous step's species sweep count
   Layout lay;
   Tuning tune;
@@ -1371,6 +1440,7 @@ int uc_create_batch(const uc_scales* scales, int64_t max_cells, int32_t max_grid
   HIPCHK(hipMalloc(&ctx->res_part, sizeof(double) * 2 * mt));
   HIPCHK(hipMalloc(&ctx->rows, sizeof(double) * 128 * max_grids));
   HIPCHK(hipHostMalloc(&ctx->rows_h, sizeof(double) * 128 * max_grids, hipHostMallocDefault));
+  HIPCHK(hipMalloc(&ctx->hist_dev, sizeof(double) * UC_HIST * max_grids));
   HIPCHK(hipMalloc(&ctx->counters, sizeof(unsigned long long) * 2 * max_grids));
   HIPCHK(hipMalloc(&ctx->fails, sizeof(int32_t) * 3 * max_grids));
   if (UC_SWEEP_LDS * 512 > 65536) {
@@ -1389,6 +1459,7 @@ int uc_create_batch(const uc_scales* scales, int64_t max_cells, int32_t max_grid
                              (int)mw_lds_bytes(4, UC_MW4_LDS)));
 
   for (auto& e : ctx->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDefault));
+  for (auto& e : ctx->tev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDefault));
   HIPCHK(hipStreamCreateWithFlags(&ctx->gs, hipStreamNonBlocking));  // default priority
   *out = ctx.release();
   return UC_OK;
@@ -1411,7 +1482,7 @@ int uc_destroy(uc_context* ctx) {
                   (void*)ctx->pk.stnd, (void*)ctx->pk.qdot, (void*)ctx->pk.qdsp, (void*)ctx->pk_iskip,
                   (void*)ctx->nsel, (void*)ctx->cpl_it, (void*)ctx->cpl_fpz, (void*)ctx->cpl_rd,
                   (void*)ctx->cpl_coef, (void*)ctx->cpl_flux, (void*)ctx->cpl_snap, (void*)ctx->cpl_zsnap,
-                  (void*)ctx->cpl_dfirst})
+                  (void*)ctx->cpl_dfirst, (void*)ctx->hist_dev})
     if (p) (void)hipFree(p);
   if (ctx->nsel_h) (void)hipHostFree(ctx->nsel_h);
   if (ctx->gs) (void)hipStreamSynchronize(ctx->gs);
@@ -1420,6 +1491,7 @@ int uc_destroy(uc_context* ctx) {
   if (ctx->gt) (void)hipHostFree(ctx->gt);
   if (ctx->rows_h) (void)hipHostFree(ctx->rows_h);
   for (auto& e : ctx->ev) (void)hipEventDestroy(e);
+  for (auto& e : ctx->tev) (void)hipEventDestroy(e);
   if (ctx->gs) (void)hipStreamDestroy(ctx->gs);
   delete ctx;
   return UC_OK;
@@ -1595,6 +1667,13 @@ int uc_sweep_residuals(const uc_context* ctx, int32_t grid, double* out, int32_t
   if (!ctx || cap < 0 || (cap > 0 && !out)) return fail(UC_E_ARG, "uc_sweep_residuals: bad argument");
   if (grid < 0 || grid >= (int32_t)ctx->run.hist.size())
     return fail(UC_E_ARG, "uc_sweep_residuals: no step of grid " + std::to_string(grid));
+  if (ctx->run.hist_dev) {
+    const int32_t n = ctx->gt[grid].hcount;
+    if (cap > 0 && n > 0)
+      HIPCHK(hipMemcpy(out, ctx->hist_dev + (int64_t)grid * UC_HIST, sizeof(double) * std::min(cap, n),
+                       hipMemcpyDeviceToHost));
+    return n;
+  }
   const std::vector<double>& h = ctx->run.hist[grid];
   for (int32_t i = 0; i < cap && i < (int32_t)h.size(); ++i) out[i] = h[i];
   return (int32_t)h.size();

@@ -366,6 +366,7 @@ void solvesp_(int32_t* isor1, double* relxsp, double* tolrsp, double* spnorm) {
   G.sor2 = 1.0 - *relxsp;
   G.tolrsp = *tolrsp;
   G.isor1 = *isor1;
+  G.spnorm = *spnorm;
   const size_t b1 = sizeof(double) * shim::LE;
   if (R.cpl_sp) {  // the closed iterates of every cell are in the field-layout iterate array
     check(ph_species_cpl(g.ctx), "solvesp_");
@@ -431,6 +432,7 @@ void solveh_(int32_t* isor2, double* relxh, double* tolrh, double* sigsp, double
   G.zsor2 = 1.0 - *relxh;
   G.tolrh = *tolrh;
   G.isor2 = *isor2;
+  G.hnorm = *hnorm;
   if (alone) {
     if (R.cpl_flux && !(*sigsp > 0.0)) stop("solveh_", "SIGSP must be positive");
     R.cp.sigsp = *sigsp;

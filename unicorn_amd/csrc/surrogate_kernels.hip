@@ -224,8 +224,8 @@ __device__ __forceinline__ bool part_all(bool b) {
 }
 
 // Statistics of the gate: stats [0] accepted, [1] out of range, [2] residual, [3]
-// sum, [4] active, [5..7] max |raw C/H/O residual| (bits of a double; non-negative
-// doubles order as their bits).  A block sums its lanes' counts in LDS and adds to
+// sum, [4] active, [5..7] max |raw C/H/O residual| over the accepted cells (bits of a
+// double; non-negative doubles order as their bits).  A block sums its lanes' counts in LDS and adds to
 // stats once per counter, since same-address atomics from every wave serialise in L2.
 struct GateCount {
   unsigned int n[5] = {0, 0, 0, 0, 0};
@@ -233,8 +233,9 @@ struct GateCount {
   __device__ void add(int why, const double* rr) {
 #pragma unroll
     for (int k = 0; k < 5; ++k) n[k] += why == k;
+    if (why == 0)   // the raw residual of the predictions the gate accepts
 #pragma unroll
-    for (int e = 0; e < 3; ++e) rmax[e] = fmax(rmax[e], fabs(rr[e]));
+      for (int e = 0; e < 3; ++e) rmax[e] = fmax(rmax[e], fabs(rr[e]));
   }
 };
 __device__ __forceinline__ void gate_flush(const GateCount& gc, unsigned long long* s_st,
