@@ -75,9 +75,19 @@ def chunked_sum(var, terms, L, indent="    ", init=True):
             L.append(f"{indent}UC_SB();")
 
 
+# uc_rates: reactions between scheduling barriers, and the Gibbs terms -G/RT kept in
+# registers (1) or stored to the slab and reloaded for each reverse rate (0).  Measured on
+# the 1001x401 field: 0.855 ms per k_rates launch with (1, 0), 0.713 with (1, 1), 0.691
+# with (4, 1), the same bits throughout.
+RATES_SB = int(os.environ.get("UC_RATES_SB", "4"))
+RATES_GREG = int(os.environ.get("UC_RATES_GREG", "1"))
+
+
 def emit_rates(t, L):
     R = t["reactions"]
     sp = t["species"]
+    gst = (lambda k, v: f"  g[{k}] = {v};") if RATES_GREG else (lambda k, v: f"  rc.st(UC_SLOT_G + {k}, {v});")
+    gld = (lambda k: f"g[{k}]") if RATES_GREG else (lambda k: f"rc.ld(UC_SLOT_G + {k})")
     L.append("// rate coefficients of one cell; writes RC[0..543] through rc.st(slot, value)")
     L.append("template <class RcPtr>")
     L.append("__device__ __forceinline__ void uc_rates(UcP P, double tk, double rho,")
@@ -90,6 +100,8 @@ def emit_rates(t, L):
     L.append("  const double t1 = T / 2.0, t2 = T * T / 6.0, t3 = t1 * t2;")
     L.append(f"  const double t4 = {f32('1.2')} * t1 * t3;")
     L.append("  const double lm1 = lnT - 1.0;")
+    if RATES_GREG:
+        L.append("  double g[52];")
     for k, s in enumerate(sp):
         lo = [hx(v) for v in s["nasa7"][:7]]
         hi_ = [hx(v) for v in s["nasa7"][7:]]
@@ -97,8 +109,8 @@ def emit_rates(t, L):
             c = [f"{v}" for v in lo]
         else:
             c = [f"(hi ? {h} : {l})" for l, h in zip(lo, hi_)]
-        L.append(f"  rc.st(UC_SLOT_G + {k}, {c[0]} * lm1 + {c[1]} * t1 + {c[2]} * t2 + {c[3]} * t3 + {c[4]} * t4"
-                 f" - {c[5]} * invT + {c[6]});")
+        L.append(gst(k, f"{c[0]} * lm1 + {c[1]} * t1 + {c[2]} * t2 + {c[3]} * t3 + {c[4]} * t4"
+                        f" - {c[5]} * invT + {c[6]}"))
         if k % 4 == 3:
             L.append("  UC_SB();")
     L.append(f"  const double lnRUT = log({f32('82.06')} * T);")
@@ -122,14 +134,14 @@ def emit_rates(t, L):
     L.append("  const double rho2 = rho * rho, rho3 = rho2 * rho;")
     L.append("  const double dmolea = rho * P->rstr / P->wmstr * 1.0e-06;")
     rhom = {1: "rho", 2: "rho2", 3: "rho3"}
-    for r in R:
+    for ri, r in enumerate(R):
         f = r["f"] - 1
         A, b, E = hx(r["A"]), hx(r["b"]), hx(r["E_R"])
         L.append(f"  {{ // slots {f + 1}/{f + 2}")
         L.append(f"    double kf = {A} * exp({b} * lnT - {E} * invT);")
         if r["rev"]:
-            num = " + ".join(f"rc.ld(UC_SLOT_G + {k - 1})" for k in r["rev"]["gsp_num"])
-            den = " + ".join(f"rc.ld(UC_SLOT_G + {k - 1})" for k in r["rev"]["gsp_den"])
+            num = " + ".join(gld(k - 1) for k in r["rev"]["gsp_num"])
+            den = " + ".join(gld(k - 1) for k in r["rev"]["gsp_den"])
             dn = r["rev"]["dn"]
             rut = {0: "", 1: " + lnRUT", -1: " - lnRUT"}[dn]
             L.append(f"    double kr = kf * exp(({den}) - ({num}){rut});")
@@ -170,7 +182,8 @@ def emit_rates(t, L):
                 e += " * y41"
             L.append(f"    rc.st({slot}, {e});")
         L.append("  }")
-        L.append("  UC_SB();")
+        if (ri + 1) % RATES_SB == 0 or ri + 1 == len(R):
+            L.append("  UC_SB();")
     L.append("}")
     L.append("")
 
