@@ -266,6 +266,15 @@ int uc_cells_changed(uc_context *ctx);
  * loop before CKRATES. */
 int uc_clip(const uc_field *field, void *stream);
 int uc_set_clip(uc_context *ctx, int32_t on);
+
+/* on = 1: the context keeps every cell's rate coefficients of the last step that advanced it
+ * (as /DUMR/ RC keeps CKRATES' output, ckrates.f:94-1333; zero before the first such step),
+ * about 4.4 KB of device memory per cell, and each step also forms QDOT and QDSP of the interior
+ * cells with 3 <= ISKIP < 10, which SOLVEH evaluates from those coefficients and SPGSOLV's
+ * prologue copies of the species although CKRATES and SOLVESP skip them (solveh.f:119-1142).
+ * on = 0 (the default): those cells' QDOT and QDSP are left as they are.  The Fortran
+ * drop-in runs with it on. */
+int uc_keep_rates(uc_context *ctx, int32_t on);
 int uc_get_tuning(const uc_context *ctx, uc_tuning *tuning);
 
 /* Per-cell status of the last step's T(h) (the conditions behind SOLVEH's messages,

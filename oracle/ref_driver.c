@@ -200,8 +200,9 @@ static void clip_closure(void);
  * vtc[n], tmu[n], vdf[52][n], stdf[n], u[n], v[n], ygeo[n], xxc[ni], xxs[ni], yyc[nj],
  * yys[nj] (field layout, j fastest).  Output of "asm": ekt[n], emu[n], coef[204][n]
  * (/DUMMY/ then /DUM0/), stc[4][n] (STE..STS), zc[4][n] (ZE..ZS); of "spg": isor1, isor2
- * (int64), resdc, resdh, then y[52][n], tk, ht, stmf, stnd, qdot. */
-static int run_assemble(const char *inp, const char *outp, int full, int ncalls) {
+ * (int64), resdc, resdh, then y[52][n], tk, ht, stmf, stnd, qdot, qdsp[52][n].  iskip2 (mode
+ * "spg", optional): ISKIP[n] (int32, field layout) from the second call on. */
+static int run_assemble(const char *inp, const char *outp, int full, int ncalls, const char *iskip2) {
     if (!spgsolv_) die("this driver has no SPGSOLV (build ref_driver_asm)");
     FILE *f = fopen(inp, "rb");
     if (!f) die("cannot open input");
@@ -262,7 +263,18 @@ static int run_assemble(const char *inp, const char *outp, int full, int ncalls)
     double relxsp = full ? 0.99 : 1.0, tolrsp = full ? 1e-8 : 1.0, resdc = 0.0;
     double relxh = full ? 0.99 : 1.0, tolrh = full ? 1e-8 : 1.0, resdh = 0.0;
     double sigsp = sc[1], sigh = sc[2];
+    int32_t *isk2 = NULL;
+    if (iskip2) {
+        FILE *g = fopen(iskip2, "rb");
+        if (!g) die("cannot open the second ISKIP array");
+        isk2 = malloc(4 * n);
+        rd(isk2, 4 * n, g);
+        fclose(g);
+    }
     for (int call = 0; call < (full ? ncalls : 1); ++call) { /* chained calls: seconds of each on stderr */
+        if (call == 1 && isk2)
+            for (int i = 0; i < ni; ++i)
+                for (int j = 0; j < nj; ++j) cb01_.iskip[fidx(j, i)] = isk2[(size_t)i * nj + j];
         if (full) {
             clip_closure();
             isor1 = 100; isor2 = 100;
@@ -290,6 +302,7 @@ static int run_assemble(const char *inp, const char *outp, int full, int ncalls)
         for (int k = 0; k < LSP; ++k) DUMPA(cb04_.fsp[k][c]);
         DUMPA(cb04_.tk[c]); DUMPA(cb04_.ht[c]); DUMPA(soot_.stmf[c]); DUMPA(soot_.stnd[c]);
         DUMPA(hrls_.qdot[c]);
+        for (int k = 0; k < LSP; ++k) DUMPA(dumd_.qdsp[k][c]);
     } else {
         DUMPA(dumy_.ekt[c]);
         DUMPA(dumy_.emu[c]);
@@ -303,9 +316,10 @@ static int run_assemble(const char *inp, const char *outp, int full, int ncalls)
 }
 
 int main(int argc, char **argv) {
-    if (argc < 3) die("usage: ref_driver mech <out> | run <in> <out> | asm <in> <out> | spg <in> <out> [ncalls]");
-    if (!strcmp(argv[1], "asm") && argc >= 4) return run_assemble(argv[2], argv[3], 0, 1);
-    if (!strcmp(argv[1], "spg") && argc >= 4) return run_assemble(argv[2], argv[3], 1, argc > 4 ? atoi(argv[4]) : 1);
+    if (argc < 3) die("usage: ref_driver mech <out> | run <in> <out> | asm <in> <out> | spg <in> <out> [ncalls [iskip2]]");
+    if (!strcmp(argv[1], "asm") && argc >= 4) return run_assemble(argv[2], argv[3], 0, 1, NULL);
+    if (!strcmp(argv[1], "spg") && argc >= 4)
+        return run_assemble(argv[2], argv[3], 1, argc > 4 ? atoi(argv[4]) : 1, argc > 5 ? argv[5] : NULL);
     if (!strcmp(argv[1], "mech")) {
         double rv[6] = {10.0, 294.0, 1.0133e+05, 0.0, 0.100, 0.233};
         setup(rv);

@@ -38,12 +38,12 @@ ASM_OUT = ("ekt", "emu", "coef", "stc", "zc")
 
 
 def spgsolv(field, tr, driver="asm", ref_values=(10.0, 294.0, 1.0133e+05, 0.0, 0.100, 0.233),
-            ncalls=1) -> dict:
+            ncalls=1, iskip_after=None) -> dict:
     """The whole of SPGSOLV after main:821-828 (ICHEM = ITHRM = 1; ISOR 100, RELX 0.99,
     TOL 1e-8): driver "asm" runs the flang-built spgsolv.f, "shim" the drop-in's spgsolv_
-    (libunicorn_fshim.so); ncalls chained calls.  Returns the last call's isor1, isor2,
-    resdc, resdh, the new y, tk, ht, stmf, stnd, qdot (field layout) and the seconds of each
-    call."""
+    (libunicorn_fshim.so); ncalls chained calls, with ISKIP iskip_after (field layout) from
+    the second call on if given.  Returns the last call's isor1, isor2, resdc, resdh, the
+    new y, tk, ht, stmf, stnd, qdot, qdsp (field layout) and the seconds of each call."""
     exe = SHIM_DRIVER if driver == "shim" else ASM_DRIVER
     if not exe.exists():
         raise FileNotFoundError(f"{exe} not built (make -C oracle asm shim)")
@@ -51,7 +51,13 @@ def spgsolv(field, tr, driver="asm", ref_values=(10.0, 294.0, 1.0133e+05, 0.0, 0
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         _write_asm_input(fin, field, tr, ref_values)
-        p = subprocess.run([str(exe), "spg", fin, fout, str(ncalls)], check=True, capture_output=True, text=True)
+        extra = []
+        if iskip_after is not None:
+            fisk = os.path.join(td, "iskip2.bin")
+            np.ascontiguousarray(iskip_after, dtype=np.int32).tofile(fisk)
+            extra = [fisk]
+        p = subprocess.run([str(exe), "spg", fin, fout, str(ncalls), *extra], check=True, capture_output=True,
+                           text=True)
         raw = np.fromfile(fout, dtype=np.uint8)
     calls = [float(l.split()[3]) for l in p.stderr.splitlines() if l.startswith("spgsolv call")]
     cnt = raw[:16].view(np.int64)
@@ -63,6 +69,8 @@ def spgsolv(field, tr, driver="asm", ref_values=(10.0, 294.0, 1.0133e+05, 0.0, 0
     for k in ("tk", "ht", "stmf", "stnd", "qdot"):
         out[k] = a[o:o + n]
         o += n
+    out["qdsp"] = a[o:o + 52 * n].reshape(52, n)
+    o += 52 * n
     assert o == a.size
     return out
 

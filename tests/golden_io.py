@@ -79,7 +79,8 @@ def assert_rates_close(qdot, qdsp_cells, ref_qdot, ref_qdsp_cells, inter, label=
     """QDOT at the cells `inter` and QDSP at the cells whose columns are given.
 
     per_value (one step from identical states): |d| <= REL_RATE |ref| + FLOOR_RATE (largest
-    |ref| of that species), and at most a fraction max_over of the values beyond REL_RATE
+    |ref| of that species), exact zeros of the reference reproduced with their sign, and at
+    most a fraction max_over of the values beyond REL_RATE
     relative (each such value is then below 1e-3 of its species' largest rate, by the first
     bound).  After chained steps the states themselves differ
     in the last bits, which stiff rates amplify, so there (per_value=False) the bar is
@@ -102,6 +103,10 @@ def assert_rates_close(qdot, qdsp_cells, ref_qdot, ref_qdsp_cells, inter, label=
     over = dk > REL_RATE * np.abs(ref_qdsp_cells)
     assert over.sum() <= max(2, max_over * over.size), \
         f"{label}: QDSP {int(over.sum())} of {over.size} values beyond 1e-10 relative"
+    # exact zeros (species 50..52, whose PRD the reference sets to 0.0) with the same sign
+    zero = ref_qdsp_cells == 0.0
+    assert np.array_equal(qdsp_cells[zero], ref_qdsp_cells[zero]) and \
+        np.array_equal(np.signbit(qdsp_cells[zero]), np.signbit(ref_qdsp_cells[zero])), f"{label}: QDSP zeros"
 
 
 def th_messages(thflag, iskip, ht, li, lj):

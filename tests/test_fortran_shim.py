@@ -156,3 +156,39 @@ def test_shim_spgsolv_matches_reference(name):
                        "spgsolv_")
     sc = np.abs(ref["qdot"][inter]).max()
     assert np.all(np.abs(got["qdot"][inter] - ref["qdot"][inter]) <= 1e-10 * sc), "spgsolv_ QDOT"
+
+
+@pytest.mark.gpu
+def test_shim_stale_rate_cells_match_reference():
+    """Cells with 3 <= ISKIP < 10: CKRATES and SOLVESP skip them (ckrates.f:94,
+    solvesp.f:89), but SOLVEH still forms their QDOT and QDSP (solveh.f:119-1142), from the
+    species SPGSOLV's prologue copies and from whatever /DUMR/ RC holds -- the coefficients
+    of the last call that advanced the cell, zero if none did.  Three chained spgsolv_ calls
+    on the assemble fixture: a twentieth of the active cells are advanced by the first call
+    and skipped (ISKIP 5) by the later ones, another twentieth are skipped (ISKIP 4) from the
+    start.  QDOT and QDSP of both sets against the flang-built SPGSOLV, per value; zero
+    values bit for bit, sign included."""
+    from golden_io import assert_rates_close, load_transport
+    if not (refrun.available("shim") and refrun.available("asm")):
+        pytest.skip("reference SPGSOLV and shim drivers not built")
+    f0, z = load("assemble")
+    tr = load_transport(z)
+    rng = np.random.default_rng(4)
+    act = f0.active()
+    pick = rng.choice(act, size=2 * (len(act) // 20), replace=False)
+    later, never = np.sort(pick[: len(pick) // 2]), np.sort(pick[len(pick) // 2:])
+    f1 = f0.copy()
+    f1.iskip[never] = 4
+    isk2 = f1.iskip.copy()
+    isk2[later] = 5
+    ref = refrun.spgsolv(f1.copy(), tr, driver="asm", ncalls=3, iskip_after=isk2)
+    print(f"stale cells: {len(later)} advanced once, {len(never)} never; reference sweeps", ref["isor1"],
+          ref["isor2"], flush=True)
+    got = refrun.spgsolv(f1.copy(), tr, driver="shim", ncalls=3, iskip_after=isk2)
+    assert (got["isor1"], got["isor2"]) == (ref["isor1"], ref["isor2"])
+    assert np.abs(ref["qdsp"][:, later]).max() > 0.0, "the kept coefficients must matter"
+    assert_rates_close(got["qdot"], got["qdsp"][:, later], ref["qdot"], ref["qdsp"][:, later], later,
+                       "cells advanced once, then skipped")
+    for k in ("qdot", "qdsp"):
+        a, b = got[k][..., never], ref[k][..., never]
+        assert np.array_equal(a, b) and np.array_equal(np.signbit(a), np.signbit(b)), k

@@ -71,7 +71,8 @@ class UcTransport(C.Structure):
 EXPORTS = ("uc_abi_version", "uc_last_error", "uc_scales_from_card", "uc_create", "uc_create_batch",
            "uc_destroy", "uc_step", "uc_step_host", "uc_step_batch", "uc_autotune", "uc_set_tuning",
            "uc_get_tuning", "uc_cells_changed", "uc_clip", "uc_set_clip", "uc_step_coupled", "uc_assemble",
-           "uc_transport_props", "uc_transport_props_noalloc", "uc_th_flags", "uc_sweep_residuals")
+           "uc_transport_props", "uc_transport_props_noalloc", "uc_th_flags", "uc_sweep_residuals",
+           "uc_keep_rates")
 
 _lib = None
 
@@ -109,6 +110,7 @@ def library() -> C.CDLL:
     lib.uc_cells_changed.argtypes = [C.c_void_p]
     lib.uc_clip.argtypes = [C.POINTER(UcField), C.c_void_p]
     lib.uc_set_clip.argtypes = [C.c_void_p, C.c_int32]
+    lib.uc_keep_rates.argtypes = [C.c_void_p, C.c_int32]
     lib.uc_destroy.argtypes = [C.c_void_p]
     lib.uc_step.argtypes = [C.c_void_p, C.POINTER(UcField), C.POINTER(UcControls), C.c_void_p,
                             C.POINTER(UcStats)]
@@ -396,6 +398,11 @@ class ChemStep:
     def set_clip(self, on: bool):
         """Whether this context's steps start with main:821-828 (default on)."""
         _check(self.lib.uc_set_clip(self.h, 1 if on else 0))
+
+    def keep_rates(self, on: bool):
+        """Keep each cell's RC of the last step that advanced it and form QDOT/QDSP of the
+        cells with 3 <= ISKIP < 10 from it, as SOLVEH does from /DUMR/ (default off)."""
+        _check(self.lib.uc_keep_rates(self.h, 1 if on else 0))
 
     def th_flags(self, dev, grid: int = 0):
         """The last step's per-cell T(h) flags of grid `grid` (dev: its DeviceField), a

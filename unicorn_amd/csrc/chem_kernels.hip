@@ -478,6 +478,9 @@ struct SourcesArgs {
   // SOLVEH without SOLVESP (ICHEM <= 0, spgsolv.f:1291-1298): the species and soot scalars
   // as given, [54][n] field layout (SSP 1..52, SST1, SST2); null: the sweeps' iterates
   const double* given;
+  // cells the sweeps did not advance (3 <= ISKIP < 10): the species of the field, y [52][n]
+  // with STMF and STND, which SPGSOLV's prologue copies into SSP/SST (spgsolv.f:103-231)
+  const double* fy;
 };
 
 // Iterates at the decided count, SST2 completion, N2 closure (solvesp.f:2839-2914)
@@ -502,6 +505,11 @@ __global__ void __launch_bounds__(WAVE) k_sources(const UcParams* __restrict__ P
   if (a.given) {
 #pragma unroll
     for (int k = 0; k < 54; ++k) s[k] = a.given[(int64_t)k * n + c];
+  } else if (a.fy) {
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) s[k] = a.fy[(int64_t)k * n + c];
+    s[52] = stmf;
+    s[53] = stnd;
   } else {
     const int32_t w = a.nsw[q];
     const int32_t m = w & NSW_MASK;
@@ -576,19 +584,36 @@ __global__ void __launch_bounds__(WAVE) k_sources(const UcParams* __restrict__ P
     const double hssp = (pk[0] * T + pk[1] * T2 / 2.0 + pk[2] * T3 / 3.0 + pk[3] * T4 / 4.0
                          + pk[4] * T5 / 5.0) / Q->wm[k] * hconvr;
     const double henth = hssp * Q->wmstr * Q->hstr + pk[5] / Q->wm[k] * Q->gasc;
-    const double prd = Q->wm[k] * Q->hfo[k] * pr;
+    // PRD(50..52) = 0.0 as written (solveh.f:1129-1131), not WM*HFO*0.0: the sign of a zero
+    // QDSP follows the reference's expression
+    const double prd = k >= 49 ? 0.0 : Q->wm[k] * Q->hfo[k] * pr;
     prod += prd;
     hrr += prd * henth / Q->hfo[k];
     QD.st(k * n, -rconv * prd / Q->hfo[k]);
   });
   rhsz = rhsz + rho * ht - dt * prod;
-  a.rhsz[q] = rhsz;
   a.qdot[c] = -hrr * rconv;
+  if (!a.rhsz) return;  // QDOT/QDSP only (cells the step does not advance)
+  a.rhsz[q] = rhsz;
   const FieldBuf SA = field_buf(a.sa, a.npad * 8 * NSP, q);
 #pragma unroll
   for (int k = 0; k < NSP; ++k) SA.st(k * a.npad, s[k]);
   a.sst[q] = s[52];
   a.sst[a.npad + q] = s[53];
+}
+
+// The rate coefficients RC(1..544) of the listed cells from their slabs into a field-layout
+// array rcf [544][n], which keeps each cell's coefficients of the last step that advanced it,
+// as /DUMR/ RC does for the reference (CKRATES writes RC only where ISKIP < 3, ckrates.f:94).
+__global__ void k_slab_to_rc(const double* __restrict__ slabs, const int32_t* __restrict__ cells, int64_t npad,
+                             int64_t n, double* __restrict__ rcf) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npad) return;
+  bool valid;
+  const int64_t c = list_cell(cells, q, valid);
+  if (!valid) return;
+  const double* tile = slabs + (q / WAVE) * (int64_t)SLAB_SLOTS * WAVE + (q % WAVE);
+  for (int r = 0; r < NRX; ++r) rcf[(int64_t)r * n + c] = tile[(int64_t)r * WAVE];
 }
 
 
@@ -1216,6 +1241,14 @@ struct uc_context {
   double* rows = nullptr;       // [max_grids][128] row sums
   double* rows_h = nullptr;     // pinned host copy
   double* hist_dev = nullptr;   // [max_grids][UC_HIST] species RSD/SPNORM per sweep (k_decide_sp)
+  // uc_keep_rates: every cell's RC of the last step that advanced it ([544][max_cells], zero
+  // until then), and the list of cells with 3 <= ISKIP < 10 whose QDOT/QDSP SOLVEH forms from it
+  bool keep_rc = false;
+  double* rc_keep = nullptr;
+  int32_t *stale_cells = nullptr, *stale_gid = nullptr;
+  double* stale_slabs = nullptr;
+  int64_t stale_npad = 0, stale_cap = 0;
+  int64_t lay_builds = 0, stale_builds = -1;  // layouts built; the layout the stale list follows
   unsigned long long* counters = nullptr;  // [2][max_grids] sweeps executed, cells re-run
   int32_t* fails = nullptr;     // [3][max_grids] T(h): cells out of limit, no root, first without root
   hipEvent_t ev[8]{};
@@ -1482,7 +1515,8 @@ int uc_destroy(uc_context* ctx) {
                   (void*)ctx->pk.stnd, (void*)ctx->pk.qdot, (void*)ctx->pk.qdsp, (void*)ctx->pk_iskip,
                   (void*)ctx->nsel, (void*)ctx->cpl_it, (void*)ctx->cpl_fpz, (void*)ctx->cpl_rd,
                   (void*)ctx->cpl_coef, (void*)ctx->cpl_flux, (void*)ctx->cpl_snap, (void*)ctx->cpl_zsnap,
-                  (void*)ctx->cpl_dfirst, (void*)ctx->hist_dev})
+                  (void*)ctx->cpl_dfirst, (void*)ctx->hist_dev, (void*)ctx->rc_keep, (void*)ctx->stale_cells,
+                  (void*)ctx->stale_gid, (void*)ctx->stale_slabs})
     if (p) (void)hipFree(p);
   if (ctx->nsel_h) (void)hipHostFree(ctx->nsel_h);
   if (ctx->gs) (void)hipStreamSynchronize(ctx->gs);
@@ -1646,6 +1680,20 @@ int uc_set_tuning(uc_context* ctx, const uc_tuning* t) {
 int uc_cells_changed(uc_context* ctx) {
   if (!ctx) return fail(UC_E_ARG, "uc_cells_changed: null context");
   ctx->lay.ngrids = 0;
+  return UC_OK;
+}
+
+int uc_keep_rates(uc_context* ctx, int32_t on) {
+  if (!ctx) return fail(UC_E_ARG, "uc_keep_rates: null context");
+  if (on && !ctx->rc_keep) {
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t bytes = sizeof(double) * NRX * (size_t)ctx->max_cells;
+    HIPCHK(hipMalloc(&ctx->rc_keep, bytes));
+    HIPCHK(hipMemset(ctx->rc_keep, 0, bytes));   // /DUMR/ RC before any CKRATES: zero
+    HIPCHK(hipDeviceSynchronize());
+  }
+  ctx->keep_rc = on != 0;
+  ctx->stale_builds = -1;
   return UC_OK;
 }
 

@@ -163,6 +163,9 @@ void ensure_context() {
   g.sc = sc;
   g.q_resident = false;
   check(uc_create(&g.sc, shim::LE, &g.ctx), "ckrates_");
+  // /DUMR/ RC as the reference keeps it: SOLVEH forms QDOT/QDSP of 3 <= ISKIP < 10 cells
+  // from the coefficients of the last step that advanced them (solveh.f:119, ckrates.f:94)
+  check(uc_keep_rates(g.ctx, 1), "ckrates_");
   if (!g.s) {
     hipcheck(hipStreamCreate(&g.s), "ckrates_: hipStreamCreate");
     const size_t b1 = sizeof(double) * shim::LE, b52 = b1 * shim::LSP;
