@@ -773,7 +773,18 @@ PRD_PREFETCH = int(os.environ.get("UC_PRD_PREFETCH", "24"))
 PRD_LDS = int(os.environ.get("UC_PRD_LDS", "72"))      # LDS positions (doubles per lane): 4 waves a CU
 
 
+# uc_prd's order: "reaction" forms each RA from one slab load and adds it to every species that
+# lists it (537 loads a cell, k_sources 0.63 ms a step on the 1001x401 field); "species" sums
+# each PRD in the reference's term order through a register+LDS cache of RA values (1,056 loads,
+# 0.79 ms).  Prefetch depth 8..32 and barrier spacing 2..8 measured within 0.02 ms.
+PRD_ORDER = os.environ.get("UC_PRD_ORDER", "reaction")
+PRD_RX_PREFETCH = int(os.environ.get("UC_PRD_RX_PREFETCH", "16"))
+PRD_RX_BARRIER = int(os.environ.get("UC_PRD_RX_BARRIER", "4"))
+
+
 def emit_prd(t, L):
+    if PRD_ORDER == "reaction":
+        return emit_prd_rx(t, L)
     reac = reactants(t)
     L.append("// net production PRD_k (solveh.f:121-1131) with the final iterates s[0..51]:")
     L.append("// pr[k] = sum over reactions of +-RA (RA = RC*prod S); PRD_k = WM_k HFO_k pr[k].")
@@ -806,6 +817,46 @@ def emit_prd(t, L):
         S.append(f"emit(P, {k - 1}, acc{k});")
     fac = {sl - 1: "".join(f" * s[{j - 1}]" for j in rs) for sl, rs in reac.items()}
     n, nl, ng = emit_cached(S, L, PRD_CACHE, PRD_PREFETCH, PRD_LDS, transform=lambda slot, v: v + fac[slot])
+    L.append("}")
+    L.append(f"// RA reads: {n} written, {nl} from LDS, {ng} loaded")
+    L.append("")
+
+
+def emit_prd_rx(t, L):
+    """uc_prd in reaction order: each RA = RC prod S is formed once from one slab load and
+    added to the accumulator of every species whose PRD lists it, with its sign
+    (solveh.f:117-1131); the 49 accumulators stay in registers.  The sums are the
+    reference's terms in another order."""
+    reac = reactants(t)
+    prd = t["prd"]
+    uses = {}    # RA slot -> [(species, sign)] in species order
+    live = []
+    for k in range(1, 50):
+        any_ = False
+        for term in prd.get(str(k), []):
+            if term["kind"] != "RA" or term["idx"] not in reac:
+                continue
+            uses.setdefault(term["idx"], []).append((k, term["sign"]))
+            any_ = True
+        if any_:
+            live.append(k)
+    L.append("// net production PRD_k (solveh.f:121-1131) with the final iterates s[0..51], in reaction")
+    L.append("// order: RA(r) = RC(r) * prod S formed once, added to each species that lists it; PRD_k =")
+    L.append("// WM_k HFO_k pr[k] (emit in species order).  No LDS.")
+    L.append("#define UC_PRD_LDS 0")
+    L.append("template <class RcPtr, class Emit>")
+    L.append("__device__ __forceinline__ void uc_prd(UcP P, RcPtr rc, const double (&s)[54], double* lds, Emit emit) {")
+    L.append("  " + " ".join(f"double acc{k} = 0.0;" for k in live))
+    S = []
+    for sl in sorted(uses):
+        ops = " ".join(f"acc{k} = acc{k} {sg} ra;" for k, sg in uses[sl])
+        S.append(f"{{ const double ra = rc.ld({sl - 1}); {ops} }}")
+    fac = {sl - 1: "".join(f" * s[{j - 1}]" for j in rs) for sl, rs in reac.items()}
+    n, nl, ng = emit_cached(S, L, 0, PRD_RX_PREFETCH, 0, transform=lambda slot, v: v + fac[slot],
+                            barrier_every=PRD_RX_BARRIER)
+    for k in range(1, NSP + 1):
+        L.append(f"  emit(P, {k - 1}, {f'acc{k}' if k in live else '0.0'});")
+    L.append("  (void)lds;")
     L.append("}")
     L.append(f"// RA reads: {n} written, {nl} from LDS, {ng} loaded")
     L.append("")
