@@ -193,11 +193,12 @@ __device__ __forceinline__ double y_after(double P, float d) {
   return u8 * u2;
 }
 
-// The gate runs four lanes per cell: lane l of a wave serves cell slot l % 16 and part
-// l / 16 of its species, k = part + 4 j (j = 0..12), whose Y0, predicted Y and tables it
+// The gate runs four lanes per cell: lane l of a wave serves cell slot l / 4 and part
+// l % 4 of its species, k = part + 4 j (j = 0..12), whose Y0, predicted Y and tables it
 // holds in registers, so Y is read from HBM once and written once.  Sums over species
-// are each part's own in index order, then combined across the four lanes as
-// (p0 + p1) + (p2 + p3), which every lane of the cell evaluates to the same bits.
+// are each part's own in index order, then combined across the cell's quad of lanes as
+// (p0 + p1) + (p2 + p3) by DPP moves, which every lane of the cell evaluates to the same
+// bits.
 constexpr int GPART = 4, GSP = NSPS / GPART;  // parts per cell, species per part
 constexpr int GATE_CELLS = GATE_BLOCK / 64 * 16;  // cells per block and round
 
@@ -208,18 +209,30 @@ struct GateSp {
   float tsd, tmean, tscale;
 };
 
+// The value of another lane of the same quad (DPP quad_perm CTRL: 0xB1 lane ^ 1, 0x4E
+// lane ^ 2, 0xFF lane 3 of the quad); a VALU move, no LDS round trip.
+template <int CTRL>
+__device__ __forceinline__ int quad_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double quad_d(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)quad_i<CTRL>((int)(uint32_t)b), hi = (uint32_t)quad_i<CTRL>((int)(uint32_t)(b >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
 __device__ __forceinline__ double part_sum(double v) {
-  v += __shfl_xor(v, 16);
-  return v + __shfl_xor(v, 32);
+  v += quad_d<0xB1>(v);
+  return v + quad_d<0x4E>(v);
 }
 __device__ __forceinline__ double part_max(double v) {
-  v = fmax(v, __shfl_xor(v, 16));
-  return fmax(v, __shfl_xor(v, 32));
+  v = fmax(v, quad_d<0xB1>(v));
+  return fmax(v, quad_d<0x4E>(v));
 }
 __device__ __forceinline__ bool part_all(bool b) {
   int v = b ? 1 : 0;
-  v &= __shfl_xor(v, 16);
-  v &= __shfl_xor(v, 32);
+  v &= quad_i<0xB1>(v);
+  v &= quad_i<0x4E>(v);
   return v != 0;
 }
 
@@ -297,9 +310,9 @@ __global__ void __launch_bounds__(GATE_BLOCK) __attribute__((amdgpu_waves_per_eu
   __shared__ unsigned long long s_st[8];
   if (threadIdx.x < 8) s_st[threadIdx.x] = 0;
   __syncthreads();
-  const int l = threadIdx.x % 64, part = l / 16;
+  const int l = threadIdx.x % 64, part = l % 4;
   GateCount gc;
-  const int64_t q0 = (int64_t)blockIdx.x * GATE_CELLS + (threadIdx.x / 64) * 16 + l % 16;
+  const int64_t q0 = (int64_t)blockIdx.x * GATE_CELLS + (threadIdx.x / 64) * 16 + l / 4;
   const bool live = q0 < ncells;
   // a lane past the list repeats the last cell's arithmetic; its stores are skipped
   const int64_t q = live ? q0 : ncells - 1, c = cells[q];
@@ -393,7 +406,7 @@ __global__ void __launch_bounds__(GATE_BLOCK) __attribute__((amdgpu_waves_per_eu
   const double ftot = tot + stm;
   const bool over = ftot > 1.0;
   const double yn = over ? 0.0 : 1.0 - ftot;
-  const double y0n = __shfl(y0[GSP - 1], 48 + l % 16);
+  const double y0n = quad_d<0xFF>(y0[GSP - 1]);   // part 3 of this cell
   dmax = fmax(dmax, fabs(yn - y0n));
   double res = 0.0;
 #pragma unroll
