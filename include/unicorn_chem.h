@@ -50,7 +50,7 @@
 extern "C" {
 #endif
 
-#define UC_ABI_VERSION 3
+#define UC_ABI_VERSION 4
 
 enum {
     UC_OK = 0,
@@ -129,6 +129,10 @@ typedef struct {
     int32_t lanes;      /* lanes per cell of the chemistry-only sweep: 1 (thread per cell), or
                            2 or 4, one lane in each of that many waves of a workgroup, which
                            split each species block's reaction terms (not with persistent) */
+    int32_t rate_parts; /* lanes per cell of the rate-coefficient kernel: 1, 2, 4 or 8, one
+                           lane in each of that many workgroups, each forming its share of
+                           the reactions' slots (the same bits for every value); 0 (the
+                           default): from the step's cell count, up to 8 for a few tiles */
 } uc_tuning;
 
 /* What SPGSOLV hands the chemistry beyond the state (spgsolv.f:98-1290); device pointers

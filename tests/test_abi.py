@@ -60,7 +60,7 @@ def test_fortran_entry_points_declared():
 
 def test_abi_version_and_errors():
     lib = chem.library()
-    assert lib.uc_abi_version() == 3
+    assert lib.uc_abi_version() == 4
     with pytest.raises(chem.ChemError):
         chem.scales_from_card(vref=-1.0)
 

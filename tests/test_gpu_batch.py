@@ -89,6 +89,39 @@ def test_batch_equals_grids_alone_and_matches_oracle(torch_cuda):
         assert_state_close(got[g], e, f.interior(), f"batch grid {g}")
 
 
+@pytest.mark.parametrize("keep", [False, True])
+def test_batch_keeps_rates_where_not_formed(torch_cuda, keep):
+    """QDOT and QDSP of a batch's grids change only at the cells SOLVEH forms them
+    (interior, ISKIP < 3, or < 10 with kept rate coefficients); every other cell keeps the
+    caller's values, and the formed ones equal the grid stepped alone."""
+    from unicorn_amd.chem import ChemStep, Controls, DeviceField
+    made = [make(s) for s in GRIDS[:3]]
+    fields = [f for f, _ in made]
+    for f in fields:
+        f.iskip[f.iskip.size // 3: f.iskip.size // 3 + 40] = 5  # a band SOLVESP skips
+    devs = [DeviceField(f.copy()) for f in fields]
+    for d in devs:
+        d.qdot.fill_(7.0)
+        d.qdsp.fill_(-3.0)
+    ctx = ChemStep(sum(f.n for f in fields), max_grids=len(fields))
+    ctx.keep_rates(keep)
+    ctx.step_batch(devs, [Controls(dt=f.dt) for f in fields])
+    for g, f in enumerate(fields):
+        ij = np.arange(f.n)
+        i, j = ij // f.lj, ij % f.lj
+        formed = (i > 0) & (i < f.li - 1) & (j > 0) & (j < f.lj - 1) & (f.iskip < (10 if keep else 3))
+        q, qs = devs[g].qdot.cpu().numpy(), devs[g].qdsp.cpu().numpy()
+        print(f"grid {g}: {formed.sum()} cells formed of {f.n}, kept rates {keep}", flush=True)
+        assert np.all(q[~formed] == 7.0) and np.all(qs[:, ~formed] == -3.0)
+        assert np.all(q[formed] != 7.0)
+        alone = DeviceField(f.copy())
+        actx = ChemStep(f.n)
+        actx.keep_rates(keep)
+        actx.step(alone, Controls(dt=f.dt))
+        assert np.array_equal(q[formed], alone.qdot.cpu().numpy()[formed])
+        assert np.array_equal(qs[:, formed], alone.qdsp.cpu().numpy()[:, formed])
+
+
 @pytest.mark.parametrize("tuning", [dict(bins=1, lpt=0, persistent=0, block=256, graph=1),
                                     dict(bins=16, lpt=1, persistent=1, block=64, graph=1),
                                     dict(bins=32, lpt=0, persistent=1, block=128, graph=1),
@@ -118,7 +151,7 @@ def test_autotune_leaves_state_and_picks_a_valid_tuning(torch_cuda):
     ctx = ChemStep(sum(f.n for f, _ in made), max_grids=len(made))
     t = ctx.autotune(devs, ctls)
     assert 1 <= t["bins"] <= 32 and t["block"] in (64, 128, 256, 512) and t["lpt"] in (0, 1) and t["persistent"] in (0, 1)
-    assert t["graph"] in (0, 1) and t["lanes"] in (1, 2, 4)
+    assert t["graph"] in (0, 1) and t["lanes"] in (1, 2, 4) and t["rate_parts"] in (0, 1, 2, 4, 8)
     assert not (t["lanes"] > 1 and t["persistent"])
     assert ctx.tuning() == t
     for b, d in zip(before, devs):

@@ -1,4 +1,5 @@
-"""Lanes per cell of the species sweep (uc_tuning.lanes), through the C ABI.
+"""Lanes per cell of the species sweep (uc_tuning.lanes) and of the rate-coefficient
+kernel (uc_tuning.rate_parts), through the C ABI.
 
 With W lanes per cell the sweep runs in W waves of one workgroup, one lane of each per
 cell: every species block's reaction terms are divided among the W waves, the partial
@@ -6,6 +7,9 @@ sums meet in LDS and every wave applies the same update.  The production and los
 are therefore added in another order than the one-thread-per-cell form adds them, so the
 split forms are held to the parity bar -- against the reference's own outputs and against
 the oracle, sweep counts exact -- and not bit for bit against the other form.
+
+The rate kernel's parts each form whole slots by the one-lane expressions, so a step with
+any rate_parts must give the one-lane step's bits.
 """
 from __future__ import annotations
 
@@ -29,7 +33,7 @@ def torch_cuda():
 def ctx_with(n, max_grids=1, **tuning):
     from unicorn_amd.chem import ChemStep
     ctx = ChemStep(n, max_grids=max_grids)
-    t = dict(bins=1, lpt=0, persistent=0, block=256, graph=1, lanes=1)
+    t = dict(bins=1, lpt=0, persistent=0, block=256, graph=1, lanes=1, rate_parts=1)
     t.update(tuning)
     ctx.set_tuning(**t)
     return ctx
@@ -98,12 +102,41 @@ def test_split_sweep_batch_matches_oracle(torch_cuda, lanes):
         assert_state_close(devs[g].to_host(), e, f.interior(), f"batch grid {g} lanes {lanes}")
 
 
+@pytest.mark.parametrize("parts", [2, 4, 8])
+def test_rate_parts_same_bits(torch_cuda, parts):
+    """Two chained steps of the soot fixture's field and a three-grid batch with the rate
+    kernel in parts: every output array equal, bit for bit, to the one-lane run's."""
+    from unicorn_amd.chem import Controls, DeviceField
+    from unicorn_amd.fields import flame_field
+    f0, _ = load("sooty")
+    fields = [f0, flame_field(45, 23, "C7H16", 12, dt_seconds=3e-6, preignition_band=0.2),
+              flame_field(27, 29, "CH4", 15, dt_seconds=2e-6)]
+    runs = []
+    for rp in (1, parts):
+        print(f"rate_parts {rp}", flush=True)
+        single = DeviceField(f0.copy())
+        ctx = ctx_with(f0.n, persistent=1, rate_parts=rp)
+        s1 = [ctx.step(single, Controls(dt=f0.dt)) for _ in range(2)]
+        devs = [DeviceField(f.copy()) for f in fields]
+        bctx = ctx_with(sum(f.n for f in fields), max_grids=len(fields), bins=16, rate_parts=rp)
+        s2 = [bctx.step_batch(devs, [Controls(dt=f.dt) for f in fields]) for _ in range(2)]
+        runs.append((single, s1, devs, s2))
+    (a, sa, da, ba), (b, sb, db, bb) = runs
+    assert [(s.isor1, s.isor2, s.resdc, s.resdh) for s in sa] == [(s.isor1, s.isor2, s.resdc, s.resdh) for s in sb]
+    assert [[(g.isor1, g.isor2) for g in s] for s in ba] == [[(g.isor1, g.isor2) for g in s] for s in bb]
+    for x, y in [(a, b)] + list(zip(da, db)):
+        for k in ("rho", "y", "tk", "ht", "stmf", "stnd", "qdot", "qdsp"):
+            u, v = getattr(x, k), getattr(y, k)
+            assert torch_cuda.equal(u, v), (parts, k)
+
+
 def test_lanes_refused_out_of_range(torch_cuda):
     from unicorn_amd.chem import ChemError, ChemStep
     ctx = ChemStep(1000)
-    for bad in (dict(lanes=3, persistent=0), dict(lanes=2, persistent=1), dict(lanes=8, persistent=0)):
-        t = dict(bins=1, lpt=0, persistent=0, block=256, graph=1, lanes=1)
+    for bad in (dict(lanes=3, persistent=0), dict(lanes=2, persistent=1), dict(lanes=8, persistent=0),
+                dict(rate_parts=3), dict(rate_parts=16), dict(rate_parts=-1)):
+        t = dict(bins=1, lpt=0, persistent=0, block=256, graph=1, lanes=1, rate_parts=1)
         t.update(bad)
         with pytest.raises(ChemError):
             ctx.set_tuning(**t)
-    assert ctx.tuning()["lanes"] == 1
+    assert ctx.tuning()["lanes"] == 1 and ctx.tuning()["rate_parts"] == 0

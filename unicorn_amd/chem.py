@@ -48,7 +48,7 @@ class UcStats(C.Structure):
 
 class UcTuning(C.Structure):
     _fields_ = [("bins", C.c_int32), ("lpt", C.c_int32), ("persistent", C.c_int32), ("block", C.c_int32),
-                ("graph", C.c_int32), ("lanes", C.c_int32)]
+                ("graph", C.c_int32), ("lanes", C.c_int32), ("rate_parts", C.c_int32)]
 
 
 class UcField(C.Structure):
@@ -387,8 +387,8 @@ class ChemStep:
         _check(self.lib.uc_autotune(self.h, fa, ca, n, C.c_void_p(stream), C.byref(t)))
         return {n_: getattr(t, n_) for n_, _ in UcTuning._fields_}
 
-    def set_tuning(self, bins=1, lpt=0, persistent=1, block=256, graph=1, lanes=1):
-        t = UcTuning(bins, lpt, persistent, block, graph, lanes)
+    def set_tuning(self, bins=1, lpt=0, persistent=1, block=256, graph=1, lanes=1, rate_parts=0):
+        t = UcTuning(bins, lpt, persistent, block, graph, lanes, rate_parts)
         _check(self.lib.uc_set_tuning(self.h, C.byref(t)))
 
     def cells_changed(self):

@@ -297,6 +297,28 @@ __global__ void __launch_bounds__(512) k_rates(const UcParams* __restrict__ Pg, 
   uc_soot(P, rc, tk, rho, a.stmf[c], a.stnd[c], tbody);
 }
 
+// The same slab in NP parts: blockIdx.y is the part, which forms the slots of its share
+// of the reactions (part 0 also rho*Y_k and the soot sources), so a cell's rates take
+// NP lanes in as many workgroups and a small cell set finishes in about 1/NP of the
+// one-lane time.  Each slot's expression is uc_rates' own: the slab is the same bits.
+template <int NP>
+__global__ void __launch_bounds__(512) k_rates_split(const UcParams* __restrict__ Pg, RatesArgs a) {
+  UcP P = (UcP)Pg;
+  const int part = blockIdx.y;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.npad) return;  // whole waves only: npad is a multiple of 64
+  bool valid;
+  const int64_t c = list_cell(a.cells, q, valid);
+  RcSlab rc = tile_slab(a.slabs, q / WAVE);
+  const FieldBuf Y = field_buf(a.y, a.n * 8 * NSP, c);
+  const double rho = a.rho[c], tk = a.tk[c];
+  double tbody;
+  if constexpr (NP == 2) uc_rates_split2(part, P, tk, rho, Y, a.n, rc, tbody);
+  else if constexpr (NP == 4) uc_rates_split4(part, P, tk, rho, Y, a.n, rc, tbody);
+  else uc_rates_split8(part, P, tk, rho, Y, a.n, rc, tbody);
+  if (part == 0) uc_soot(P, rc, tk, rho, a.stmf[c], a.stnd[c], tbody);
+}
+
 struct SweepArgs {
   const double* rho; const double* y; const double* stmf; const double* stnd;
   const int32_t* cells; const int32_t* gid; const GridTab* gt;
@@ -1052,9 +1074,12 @@ struct PackPtrs {
   const int32_t* iskip;
 };
 
-// block (x, g): cells x*256.. of grid g; dir 0 packs (grid -> packed), 1 unpacks
+// block (x, g): cells x*256.. of grid g; dir 0 packs (grid -> packed), 1 unpacks.  QDOT and
+// QDSP are formed only at interior cells with ISKIP < qlim (3, or 10 where the kept rate
+// coefficients give the cells SOLVEH alone visits theirs, solveh.f:119); elsewhere the
+// grid's arrays keep what they held, as the reference's do.
 __global__ void k_pack(const PackPtrs* __restrict__ grids, const GridTab* __restrict__ gt, PackPtrs pk,
-                       int32_t* __restrict__ pk_iskip, int64_t N, int dir) {
+                       int32_t* __restrict__ pk_iskip, int64_t N, int dir, int32_t qlim) {
   const int32_t g = blockIdx.y;
   const GridTab& G = gt[g];
   const int64_t ng = (int64_t)G.li * G.lj;
@@ -1068,10 +1093,12 @@ __global__ void k_pack(const PackPtrs* __restrict__ grids, const GridTab* __rest
     for (int k = 0; k < NSP; ++k) pk.y[k * N + d] = f.y[k * ng + c];
   } else {
     f.rho[c] = pk.rho[d]; f.tk[c] = pk.tk[d]; f.ht[c] = pk.ht[d];
-    f.stmf[c] = pk.stmf[d]; f.stnd[c] = pk.stnd[d]; f.qdot[c] = pk.qdot[d];
-    for (int k = 0; k < NSP; ++k) {
-      f.y[k * ng + c] = pk.y[k * N + d];
-      f.qdsp[k * ng + c] = pk.qdsp[k * N + d];
+    f.stmf[c] = pk.stmf[d]; f.stnd[c] = pk.stnd[d];
+    for (int k = 0; k < NSP; ++k) f.y[k * ng + c] = pk.y[k * N + d];
+    const int64_t i = c / G.lj, j = c % G.lj;
+    if (i > 0 && i < G.li - 1 && j > 0 && j < G.lj - 1 && pk_iskip[d] < qlim) {
+      f.qdot[c] = pk.qdot[d];
+      for (int k = 0; k < NSP; ++k) f.qdsp[k * ng + c] = pk.qdsp[k * N + d];
     }
   }
 }
@@ -1181,6 +1208,7 @@ struct Tuning {
   int32_t block = 256;     // block size of the one-thread-per-cell kernels
   int32_t graph = 1;       // sweep segments replayed from a captured hipGraph
   int32_t lanes = 1;       // lanes per cell of the chemistry-only sweep (1, 2, 4)
+  int32_t rate_parts = 0;  // lanes per cell of the rate-coefficient kernel (1, 2, 4, 8; 0: by cell count)
 };
 
 // the sweep segment (grid-table upload, queue reset, k_sweep, per-grid row sums and
@@ -1673,7 +1701,10 @@ int uc_set_tuning(uc_context* ctx, const uc_tuning* t) {
     return fail(UC_E_ARG, "uc_set_tuning: lanes must be 1, 2 or 4");
   if (t->lanes > 1 && t->persistent)
     return fail(UC_E_ARG, "uc_set_tuning: the split sweep (lanes > 1) runs one tile per workgroup, not persistent");
-  ctx->tune = Tuning{t->bins, t->lpt ? 1 : 0, t->persistent ? 1 : 0, t->block, t->graph ? 1 : 0, t->lanes};
+  if (t->rate_parts != 0 && t->rate_parts != 1 && t->rate_parts != 2 && t->rate_parts != 4 && t->rate_parts != 8)
+    return fail(UC_E_ARG, "uc_set_tuning: rate_parts must be 0 (by cell count), 1, 2, 4 or 8");
+  ctx->tune = Tuning{t->bins, t->lpt ? 1 : 0, t->persistent ? 1 : 0, t->block, t->graph ? 1 : 0, t->lanes,
+                     t->rate_parts};
   return UC_OK;
 }
 
@@ -1739,7 +1770,7 @@ int uc_th_flags(const uc_context* ctx, int32_t grid, uint8_t* flags, void* strea
 int uc_get_tuning(const uc_context* ctx, uc_tuning* t) {
   if (!ctx || !t) return fail(UC_E_ARG, "uc_get_tuning: null argument");
   *t = uc_tuning{ctx->tune.bins, ctx->tune.lpt, ctx->tune.persistent, ctx->tune.block, ctx->tune.graph,
-                 ctx->tune.lanes};
+                 ctx->tune.lanes, ctx->tune.rate_parts};
   return UC_OK;
 }
 

@@ -81,17 +81,29 @@ def chunked_sum(var, terms, L, indent="    ", init=True):
 # with (4, 1), the same bits throughout.
 RATES_SB = int(os.environ.get("UC_RATES_SB", "4"))
 RATES_GREG = int(os.environ.get("UC_RATES_GREG", "1"))
+# part counts of the split rate kernel (k_rates_split<NP>)
+RATES_SPLIT = (2, 4, 8)
 
 
-def emit_rates(t, L):
+def emit_rates(t, L, name="uc_rates", sel=None, rhofa=True):
+    """uc_rates: every rate coefficient of one cell, without contraction.  With sel
+    (reaction indices) the function forms only those reactions' slots, and with rhofa
+    False it leaves the rho*Y_k slots to another part; the shared preamble (T, Gibbs
+    terms, third bodies) is emitted whole and the compiler drops what the selected
+    reactions do not read."""
     R = t["reactions"]
     sp = t["species"]
     gst = (lambda k, v: f"  g[{k}] = {v};") if RATES_GREG else (lambda k, v: f"  rc.st(UC_SLOT_G + {k}, {v});")
     gld = (lambda k: f"g[{k}]") if RATES_GREG else (lambda k: f"rc.ld(UC_SLOT_G + {k})")
-    L.append("// rate coefficients of one cell; writes RC[0..543] through rc.st(slot, value)")
+    if sel is None:
+        L.append("// rate coefficients of one cell; writes RC[0..543] through rc.st(slot, value)")
     L.append("template <class RcPtr>")
-    L.append("__device__ __forceinline__ void uc_rates(UcP P, double tk, double rho,")
+    L.append(f"__device__ __forceinline__ void {name}(UcP P, double tk, double rho,")
     L.append("    const double (&fa)[52], RcPtr rc, double& tbody_out) {")
+    # each operation rounded as written, as the reference does: contraction would pick
+    # which product of a sum to fuse by how often the compiler finds it shared (CSE across
+    # reactions), and a part of the split kernel shares fewer than the whole
+    L.append("#pragma clang fp contract(off)")
     L.append("  const double T = tk * P->tstr;")
     L.append("  const double lnT = log(T);")
     L.append("  const double invT = 1.0 / T;")
@@ -128,12 +140,14 @@ def emit_rates(t, L):
             expr += f" {term['sign']} {c} * fa[{term['species'] - 1}] / P->wm[{term['species'] - 1}]"
         L.append(f"  const double tb{g} = {expr};")
     L.append("  const double tb0 = tbody;")
-    L.append("  for (int k = 0; k < 51; ++k) rc.st(UC_SLOT_RHOFA + k, rho * fa[k]);")
+    if rhofa:
+        L.append("  for (int k = 0; k < 51; ++k) rc.st(UC_SLOT_RHOFA + k, rho * fa[k]);")
     L.append("  const double y41 = fa[40];")
     L.append("  UC_SB();")
     L.append("  const double rho2 = rho * rho, rho3 = rho2 * rho;")
     L.append("  const double dmolea = rho * P->rstr / P->wmstr * 1.0e-06;")
     rhom = {1: "rho", 2: "rho2", 3: "rho3"}
+    R = [r for i, r in enumerate(R) if sel is None or i in sel]
     for ri, r in enumerate(R):
         f = r["f"] - 1
         A, b, E = hx(r["A"]), hx(r["b"]), hx(r["E_R"])
@@ -184,6 +198,57 @@ def emit_rates(t, L):
         L.append("  }")
         if (ri + 1) % RATES_SB == 0 or ri + 1 == len(R):
             L.append("  UC_SB();")
+    L.append("}")
+    L.append("")
+
+
+def rate_cost(r):
+    """Transcendental calls a reaction's slots take in emit_rates: the forward exp, the
+    reverse exp, and the falloff's low-pressure exp with its broadening (log10 of the
+    centre, the exps inside it, log10 and exp10 of the reduced pressure)."""
+    c = 1 + (1 if r["rev"] else 0)
+    fo = r["falloff"]
+    if fo:
+        c += 1
+        t1v, t2v = float.fromhex(fo["troe"][0]), float.fromhex(fo["troe"][1])
+        if t1v > 0.0:
+            c += (1 if t2v == 0.0 else 4) + 2
+        elif t1v < 0.0:
+            c += 4
+    return c
+
+
+def emit_rates_split(t, L, NP):
+    """k_rates in NP parts: part p forms the slots of its share of the reactions (its
+    own workgroups, so no exchange), part 0 also the rho*Y_k slots.  Reactions are dealt
+    to the least-loaded part by transcendental count, largest first, and keep their
+    order within a part.  Every slot is formed by the same expression as in uc_rates,
+    from the same preamble, so the slab is bit for bit the one uc_rates writes."""
+    R = t["reactions"]
+    load = [0] * NP
+    sel = [set() for _ in range(NP)]
+    for i in sorted(range(len(R)), key=lambda i: (-rate_cost(R[i]), i)):
+        p = min(range(NP), key=lambda p: (load[p], p))
+        sel[p].add(i)
+        load[p] += rate_cost(R[i])
+    L.append(f"// rate coefficients in {NP} parts (transcendentals per part: {', '.join(map(str, load))})")
+    for p in range(NP):
+        emit_rates(t, L, name=f"uc_rates_s{NP}_{p}", sel=sel[p], rhofa=p == 0)
+    L.append("// part `part` (wave-uniform) of the cell whose species Y.ld(k*n) reads")
+    L.append("template <class RcPtr>")
+    L.append(f"__device__ __forceinline__ void uc_rates_split{NP}(int part, UcP P, double tk, double rho,")
+    L.append("    FieldBuf Y, int64_t n, RcPtr rc, double& tbody_out) {")
+    L.append("  switch (part) {")
+    # each case loads the species itself, from an offset and with T and rho made opaque to
+    # the compiler, so neither the loads nor the parts' common preamble are hoisted above
+    # the switch (the union of what the parts keep would be live at once and spill)
+    for p in range(NP):
+        L.append(f"    case {p}: {{ double t = tk, r = rho; uint32_t o = Y.c8;"
+                 " asm volatile(\"\" : \"+v\"(t), \"+v\"(r), \"+v\"(o)); const FieldBuf Yp{Y.r, o};"
+                 " double fa[52]; for (int k = 0; k < 52; ++k) fa[k] = Yp.ld(k * n);"
+                 f" uc_rates_s{NP}_{p}(P, t, r, fa, rc, tbody_out); return; }}")
+    L.append("    default: return;")
+    L.append("  }")
     L.append("}")
     L.append("")
 
@@ -923,6 +988,8 @@ def main(out=None):
     emit_host_tables(t, L)
     emit_transport_tables(L)
     emit_rates(t, L)
+    for NP in RATES_SPLIT:
+        emit_rates_split(t, L, NP)
     emit_soot(L)
     emit_sweep(t, L)
     emit_sweep(t, L, coupled=True)
