@@ -125,7 +125,8 @@ typedef struct {
     int32_t lpt;        /* 1: tiles of 64 cells run stiffest first (longest-processing-time) */
     int32_t persistent; /* 1: the sweep kernel runs persistent waves over the tile queue */
     int32_t block;      /* block size of the one-thread-per-cell kernels: 64, 128, 256 or 512 */
-    int32_t graph;      /* 1: each sweep segment is replayed from a captured hipGraph */
+    int32_t graph;      /* 1: each sweep segment is replayed from a captured hipGraph (default 0:
+                           launched directly, which measured faster) */
     int32_t lanes;      /* lanes per cell of the chemistry-only sweep: 1 (thread per cell), or
                            2 or 4, one lane in each of that many waves of a workgroup, which
                            split each species block's reaction terms (not with persistent) */

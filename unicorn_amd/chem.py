@@ -387,7 +387,7 @@ class ChemStep:
         _check(self.lib.uc_autotune(self.h, fa, ca, n, C.c_void_p(stream), C.byref(t)))
         return {n_: getattr(t, n_) for n_, _ in UcTuning._fields_}
 
-    def set_tuning(self, bins=1, lpt=0, persistent=1, block=256, graph=1, lanes=1, rate_parts=0):
+    def set_tuning(self, bins=1, lpt=0, persistent=1, block=256, graph=0, lanes=1, rate_parts=0):
         t = UcTuning(bins, lpt, persistent, block, graph, lanes, rate_parts)
         _check(self.lib.uc_set_tuning(self.h, C.byref(t)))
 

@@ -110,6 +110,13 @@ __device__ __forceinline__ FieldBuf field_buf(const double* base, int64_t bytes,
 #include "chem_gen.inc"
 static_assert(UC_SLAB_SLOTS == SLAB_SLOTS, "slab layout");
 
+// The long straight-line kernels (70-380 KB of code each, larger than the instruction
+// cache) start on 64 KB boundaries of the code object.  Their speed depended on where the
+// linker happened to place them: k_sweep ran 3.5% slower at some offsets with the same
+// instructions (1001x401 field, 20 steps: 110.8 against 107.2 ms of sweeps), and at a
+// 64 KB boundary it runs at its best whatever else the code object holds.
+#define UC_HOT __attribute__((aligned(65536)))
+
 // Reads of the coupled sweep (uc_sweep_cpl): slab slots, the cell's coupling slots
 // (UC_CPL + s, a tile of UC_CPL_SLOTS per wave like the slab), and the neighbours'
 // current iterates (UC_NBR + 4*row + dir, dir 0: J+1, 1: J-1, 2: I+1, 3: I-1) from the
@@ -278,7 +285,7 @@ struct RatesArgs {
 
 // rate coefficients, rho*Y_k and soot sources of every cell into its slab
 // (ckrates.f:92-1352)
-__global__ void __launch_bounds__(512) k_rates(const UcParams* __restrict__ Pg, RatesArgs a) {
+__global__ void __launch_bounds__(512) UC_HOT k_rates(const UcParams* __restrict__ Pg, RatesArgs a) {
   UcP P = (UcP)Pg;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= a.npad) return;  // whole waves only: npad is a multiple of 64
@@ -302,7 +309,7 @@ __global__ void __launch_bounds__(512) k_rates(const UcParams* __restrict__ Pg, 
 // NP lanes in as many workgroups and a small cell set finishes in about 1/NP of the
 // one-lane time.  Each slot's expression is uc_rates' own: the slab is the same bits.
 template <int NP>
-__global__ void __launch_bounds__(512) k_rates_split(const UcParams* __restrict__ Pg, RatesArgs a) {
+__global__ void __launch_bounds__(512) UC_HOT k_rates_split(const UcParams* __restrict__ Pg, RatesArgs a) {
   UcP P = (UcP)Pg;
   const int part = blockIdx.y;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -452,7 +459,7 @@ __device__ __forceinline__ void sweep_tile(UcP P, const SweepArgs& a, double* ld
 // stiffness-sorted device work queue): one per block, or, in the persistent form,
 // as many as each wave takes from the shared queue position.
 template <bool PERSISTENT>
-__global__ void __launch_bounds__(WAVE) k_sweep(const UcParams* __restrict__ Pg, SweepArgs a) {
+__global__ void __launch_bounds__(WAVE) UC_HOT k_sweep(const UcParams* __restrict__ Pg, SweepArgs a) {
   extern __shared__ double uc_lds[];
   double* const lds = uc_lds + threadIdx.x;
   UcP P = (UcP)Pg;
@@ -474,7 +481,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const UcParams* __restrict__ Pg,
 // W lanes per cell: W waves per block, one tile per block.  Dynamic LDS: each wave's
 // UC_MW<W>_LDS positions per lane, then the exchange area [2][2W+1][64].
 template <int W>
-__global__ void __launch_bounds__(W * WAVE) k_sweep_mw(const UcParams* __restrict__ Pg, SweepArgs a) {
+__global__ void __launch_bounds__(W * WAVE) UC_HOT k_sweep_mw(const UcParams* __restrict__ Pg, SweepArgs a) {
   extern __shared__ double uc_lds[];
   constexpr int KL = W == 2 ? UC_MW2_LDS : UC_MW4_LDS;
   const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
@@ -509,7 +516,7 @@ struct SourcesArgs {
 // and the SOLVEH sources: radiation, RA/PRD, QDOT, QDSP, RHSZ (solveh.f:34-1142).
 // One wave per block: the block's dynamic LDS (UC_PRD_LDS doubles per lane) is the
 // second-level cache of the RA values uc_prd forms.
-__global__ void __launch_bounds__(WAVE) k_sources(const UcParams* __restrict__ Pg, SourcesArgs a) {
+__global__ void __launch_bounds__(WAVE) UC_HOT k_sources(const UcParams* __restrict__ Pg, SourcesArgs a) {
   extern __shared__ double uc_lds[];
   double* const lds = uc_lds + threadIdx.x;
   UcP P = (UcP)Pg;
@@ -691,7 +698,7 @@ struct FinishArgs {
   int32_t max_grids;
 };
 
-__global__ void __launch_bounds__(512) k_finish(const UcParams* __restrict__ Pg, FinishArgs a) {
+__global__ void __launch_bounds__(512) UC_HOT k_finish(const UcParams* __restrict__ Pg, FinishArgs a) {
 #pragma clang fp contract(off)   // the reference's operations in its order
   UcP P = (UcP)Pg;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -930,7 +937,7 @@ __device__ __forceinline__ bool wave_cell(const WaveArgs& a, int32_t& si, int64_
   return true;
 }
 
-__global__ void __launch_bounds__(WAVE) k_sweep_cpl(const UcParams* __restrict__ Pg, WaveArgs a) {
+__global__ void __launch_bounds__(WAVE) UC_HOT k_sweep_cpl(const UcParams* __restrict__ Pg, WaveArgs a) {
   extern __shared__ double uc_lds[];
   double* const lds = uc_lds + threadIdx.x;
   UcP P = (UcP)Pg;
@@ -1206,7 +1213,8 @@ struct Tuning {
   int32_t lpt = 0;         // tiles stiffest first
   int32_t persistent = 1;  // k_sweep as persistent waves over the tile queue
   int32_t block = 256;     // block size of the one-thread-per-cell kernels
-  int32_t graph = 1;       // sweep segments replayed from a captured hipGraph
+  int32_t graph = 0;       // sweep segments replayed from a captured hipGraph (measured slower than
+                           // direct launches: two cross-stream event waits per replay)
   int32_t lanes = 1;       // lanes per cell of the chemistry-only sweep (1, 2, 4)
   int32_t rate_parts = 0;  // lanes per cell of the rate-coefficient kernel (1, 2, 4, 8; 0: by cell count)
 };
@@ -1268,6 +1276,7 @@ struct uc_context {
   double* res_part = nullptr;   // [2][max_tiles]
   double* rows = nullptr;       // [max_grids][128] row sums
   double* rows_h = nullptr;     // pinned host copy
+  char* end_h = nullptr;        // pinned host copies of res_part, counters and fails (ph_end)
   double* hist_dev = nullptr;   // [max_grids][UC_HIST] species RSD/SPNORM per sweep (k_decide_sp)
   // uc_keep_rates: every cell's RC of the last step that advanced it ([544][max_cells], zero
   // until then), and the list of cells with 3 <= ISKIP < 10 whose QDOT/QDSP SOLVEH forms from it
@@ -1501,6 +1510,8 @@ int uc_create_batch(const uc_scales* scales, int64_t max_cells, int32_t max_grid
   HIPCHK(hipMalloc(&ctx->res_part, sizeof(double) * 2 * mt));
   HIPCHK(hipMalloc(&ctx->rows, sizeof(double) * 128 * max_grids));
   HIPCHK(hipHostMalloc(&ctx->rows_h, sizeof(double) * 128 * max_grids, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc(&ctx->end_h, sizeof(double) * 2 * (mt + 2 * max_grids) + sizeof(int32_t) * 3 * max_grids,
+                       hipHostMallocDefault));
   HIPCHK(hipMalloc(&ctx->hist_dev, sizeof(double) * UC_HIST * max_grids));
   HIPCHK(hipMalloc(&ctx->counters, sizeof(unsigned long long) * 2 * max_grids));
   HIPCHK(hipMalloc(&ctx->fails, sizeof(int32_t) * 3 * max_grids));
@@ -1552,6 +1563,7 @@ int uc_destroy(uc_context* ctx) {
   if (ctx->seg.graph) (void)hipGraphDestroy(ctx->seg.graph);
   if (ctx->gt) (void)hipHostFree(ctx->gt);
   if (ctx->rows_h) (void)hipHostFree(ctx->rows_h);
+  if (ctx->end_h) (void)hipHostFree(ctx->end_h);
   for (auto& e : ctx->ev) (void)hipEventDestroy(e);
   for (auto& e : ctx->tev) (void)hipEventDestroy(e);
   if (ctx->gs) (void)hipStreamDestroy(ctx->gs);
