@@ -132,8 +132,9 @@ typedef struct {
                            split each species block's reaction terms (not with persistent) */
     int32_t rate_parts; /* lanes per cell of the rate-coefficient kernel: 1, 2, 4 or 8, one
                            lane in each of that many workgroups, each forming its share of
-                           the reactions' slots (the same bits for every value); 0 (the
-                           default): from the step's cell count, up to 8 for a few tiles */
+                           the reactions' slots; with 8 the T(h) kernel also takes 8 lanes
+                           per cell (the same bits for every value); 0 (the default): from
+                           the step's cell count, up to 8 for a few tiles */
 } uc_tuning;
 
 /* What SPGSOLV hands the chemistry beyond the state (spgsolv.f:98-1290); device pointers

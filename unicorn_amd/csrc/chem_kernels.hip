@@ -698,6 +698,131 @@ struct FinishArgs {
   int32_t max_grids;
 };
 
+// One cell's enthalpy relaxation, T(h) and SPGSOLV epilogue as k_finish forms them, on
+// the cell's 8 adjacent lanes (part = lane % 8): lane p < 6 sums coefficient A(p+1) of
+// T(h) over the species in the reference's order (dividing by p+1 where the reference
+// does; the other sums divide by 1.0, which is exact), the sums reach the cell's other
+// lanes by shuffles, every lane runs the rest of k_finish's arithmetic and part 0 writes.
+// The results are k_finish's bits; this copy of its arithmetic exists because the
+// one-lane kernel sits at the register file's limit and any shared form of the two
+// spilled it.  limit/noroot: the conditions of solveh.f:1200-1204 and :1217-1219.
+__device__ __forceinline__ void finish_cell_p8(UcP P, const FinishArgs& a, int64_t q, int64_t c, int part,
+                                               const GridTab& G, bool& limit, bool& noroot, double& rc_, double& rh_) {
+#pragma clang fp contract(off)   // the reference's operations in its order
+  const bool writer = part == 0;
+  const int64_t n = a.n;
+  const FieldBuf SA = field_buf(a.sa, a.npad * 8 * NSP, q), Y = field_buf(a.y, n * 8 * NSP, c);
+  double s[NSP];
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) s[k] = SA.ld(k * a.npad);
+  const double rho = a.rho[c], ht = a.ht[c];
+  // enthalpy relaxation (solveh.f:1147-1167), zero transport: PPP = RHO
+  double fpz = ht;
+  if (a.fpz_in) {
+    fpz = a.fpz_in[q];
+  } else {
+    const double inc = a.rhsz[q] * G.zsor1 / rho;
+    const bool percell = G.mode == UC_MODE_PERCELL;
+    for (int32_t sw = 0; sw < G.nh; ++sw) {
+      const double fnew = G.zsor2 * fpz + inc;
+      const double d = fabs(fpz - fnew);
+      fpz = fnew;
+      if (percell && d <= G.tolrh) break;
+    }
+  }
+  // T from h (solveh.f:1170-1230); ENTH = (FPZ - (U*U+V*V+W*W)/2)*HSTR + sum Y*HFO*HSTR
+  const double uc = a.u ? a.u[c] : 0.0, vc = a.v ? a.v[c] : 0.0, wc = a.w ? a.w[c] : 0.0;
+  double enth = (fpz - 0.5 * (uc * uc + vc * vc + wc * wc)) * P->hstr;
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) enth += s[k] * P->hfo[k] * P->hstr;
+  bool hiT = false;
+  double tpoly = 1000.0;
+  double tkd = a.tk[c] * P->tstr;
+  double a1, a2, a3, a4, a5, a6;
+  for (;;) {
+    if (tkd > tpoly) tkd = tpoly / 2.0;
+    asm volatile("" : "+s"(P));
+    {
+      // A(i) = sum over k of POLSP(i,k)*Y(k)/WM(k)/i (/1 for A1 and A6: exact), lane i-1
+      const int i = part < 6 ? part : 5;
+      const double div = i == 0 || i == 5 ? 1.0 : (double)(i + 1);
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < NSP; ++k) {
+        const double wk = P->wm[k];
+        // this lane's coefficient from the row's scalar loads (i and hiT vary by lane)
+        double cf = hiT ? P->nasa[k][7] : P->nasa[k][0];
+#pragma unroll
+        for (int m = 1; m < 6; ++m) cf = i == m ? (hiT ? P->nasa[k][7 + m] : P->nasa[k][m]) : cf;
+        acc = acc + cf * s[k] / wk / div;
+      }
+      a1 = __shfl(acc, 0, 8);
+      a2 = __shfl(acc, 1, 8);
+      a3 = __shfl(acc, 2, 8);
+      a4 = __shfl(acc, 3, 8);
+      a5 = __shfl(acc, 4, 8);
+      a6 = __shfl(acc, 5, 8);
+    }
+    const double funt = ((a1 + (a2 + (a3 + (a4 + a5 * tpoly) * tpoly) * tpoly) * tpoly) * tpoly + a6)
+                        * P->gasc / P->wmstr - enth;
+    if (funt >= 1.0e-07) break;
+    if (hiT) { limit = true; break; }   // above the high-range fit at TPOL2 (message 712)
+    hiT = true;
+    if (tkd <= tpoly) tkd = 5000.0 / 2.0;
+    tpoly = 5000.0;
+  }
+  const double eh = enth * P->wmstr / P->gasc;
+  noroot = true;   // until a correction falls within the tolerance (message 714 otherwise)
+  for (int itn = 1; itn <= 100; ++itn) {
+    const double dtkd = ((a1 + (a2 + (a3 + (a4 + a5 * tkd) * tkd) * tkd) * tkd) * tkd + a6 - eh)
+                        / (a1 + (2.0 * a2 + (3.0 * a3 + (4.0 * a4 + 5.0 * a5 * tkd) * tkd) * tkd) * tkd);
+    if (fabs(dtkd) <= (double)0.001f) { noroot = false; break; }
+    tkd = tkd - dtkd;
+  }
+  if (tkd > 5000.0) tkd = 5000.0;
+  if (tkd < P->tstr - 1.0) tkd = P->tstr - 1.0;
+  const double tnew = tkd / P->tstr;
+  if (a.fpz_out) {  // SOLVEH's outputs only (solveh.f:1167, :1230)
+    if (writer) {
+      a.fpz_out[c] = fpz;
+      a.tk[c] = tnew;
+    }
+    return;
+  }
+  if (!writer) return;
+  // SPGSOLV epilogue (spgsolv.f:1299-1416)
+  const double sst1 = a.sst[q], sst2 = a.sst[a.npad + q];
+  rh_ = fabs(fpz - ht);
+  rc_ = fabs(sst1 - a.stmf[c]);
+  double rbar = 0.0;
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) {
+    if (k < 51) rc_ += fabs(s[k] - Y.ld(k * n));
+    Y.st(k * n, s[k]);
+    rbar = rbar + s[k] / P->wm[k];   // main:1058
+  }
+  a.ht[c] = fpz;
+  a.stmf[c] = sst1;
+  a.stnd[c] = sst2;
+  a.tk[c] = tnew;
+  a.rho[c] = P->pback / (P->beta1 * tnew * rbar);  // main:1056-1061
+}
+
+// T(h) failure counts of a grid and its first failing cell (field index within the grid)
+__device__ __forceinline__ void count_fails(const FinishArgs& a, int32_t g, const GridTab& G, int64_t q, bool limit,
+                                            bool noroot) {
+  const unsigned long long bl = __ballot(limit), bn = __ballot(noroot);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    if (bl) atomicAdd(&a.fails[g], (int32_t)__popcll(bl));
+    if (bn) atomicAdd(&a.fails[a.max_grids + g], (int32_t)__popcll(bn));
+  }
+  if (noroot) {
+    bool valid;
+    const int64_t c = list_cell(a.cells, q, valid);
+    atomicMin(&a.fails[2 * a.max_grids + g], (int32_t)(c - G.off));
+  }
+}
+
 __global__ void __launch_bounds__(512) UC_HOT k_finish(const UcParams* __restrict__ Pg, FinishArgs a) {
 #pragma clang fp contract(off)   // the reference's operations in its order
   UcP P = (UcP)Pg;
@@ -811,6 +936,39 @@ done:
   if ((threadIdx.x & (WAVE - 1)) == 0) {
     a.res_part[q / WAVE] = rc_;
     a.res_part[a.ntiles + q / WAVE] = rh_;
+  }
+}
+
+// The same with 8 lanes per cell for a few tiles, where one lane's chain of divisions
+// through the T(h) coefficients bounds the step: a block of 512 threads is one tile, cell
+// slot threadIdx.x / 8.  The tile's residual partials are summed by one wave in the
+// one-lane kernel's order.
+__global__ void __launch_bounds__(512) UC_HOT k_finish_p8(const UcParams* __restrict__ Pg, FinishArgs a) {
+  __shared__ double part_rc[WAVE], part_rh[WAVE];
+  UcP P = (UcP)Pg;
+  const int slot = threadIdx.x / 8, part = threadIdx.x % 8;
+  const int64_t tile = blockIdx.x;
+  const int64_t q = tile * WAVE + slot;   // grid: npad / 64 blocks, so q < npad
+  const int32_t g = wave_grid(a.gid, q);   // a tile belongs to one grid
+  const GridTab& G = a.gt[g];
+  double rc_ = 0.0, rh_ = 0.0;
+  bool limit = false, noroot = false;
+  bool valid;
+  const int64_t c = list_cell(a.cells, q, valid);
+  if (valid) finish_cell_p8(P, a, q, c, part, G, limit, noroot, rc_, rh_);
+  if (valid && part == 0) a.thflag[c] = (uint8_t)((limit ? 1 : 0) | (noroot ? 2 : 0));
+  count_fails(a, g, G, q, limit && part == 0, noroot && part == 0);
+  if (part == 0) {
+    part_rc[slot] = rc_;
+    part_rh[slot] = rh_;
+  }
+  __syncthreads();
+  if (threadIdx.x < WAVE) {
+    const double r1 = wave_sum(part_rc[threadIdx.x]), r2 = wave_sum(part_rh[threadIdx.x]);
+    if (threadIdx.x == 0) {
+      a.res_part[tile] = r1;
+      a.res_part[a.ntiles + tile] = r2;
+    }
   }
 }
 
