@@ -26,13 +26,11 @@ constexpr int NF = 54, NT = 53, NPAD = 64;
 #endif
 #define GATE_WAVES UC_GATE_WAVES  // waves per SIMD the gate's registers must allow (3 spills)
 constexpr int GATE_BLOCK = UC_GATE_BLOCK;
-// k_network: waves per block.  The block stages every weight k-step through LDS once
-// for all its waves (16 cells each), so more waves per block read the weights from L2
-// fewer times per cell.
-#ifndef UC_NET_WAVES
-#define UC_NET_WAVES 4
-#endif
-constexpr int NET_WAVES = UC_NET_WAVES;
+// k_network: waves per block (32 cells each).  The block holds half of a hidden layer's
+// weights in LDS at a time (64 KB), so two blocks share a CU and one stages while the
+// other computes: two waves per SIMD.
+constexpr int NET_WAVES = 4;
+constexpr int NET_LDS = 128 * 256 * 2;   // bytes: half a hidden layer, 128 x 256 bf16
 
 __device__ __forceinline__ double boxcox(double y) { return (pow(y > 0.0 ? y : 0.0, 0.1) - 1.0) / 0.1; }
 // inverse on the range of boxcox: (1 + 0.1 b)^10, 0 below b = -10
@@ -452,17 +450,17 @@ __global__ void __launch_bounds__(GATE_BLOCK) __attribute__((amdgpu_waves_per_eu
   gate_flush(gc, s_st, stats);
 }
 
-// ---- the network on a field: features -> MLP -> outputs, 16 cells per wave, one launch
-// (k_gate follows).
+// ---- the network on a field: features -> MLP -> outputs, 32 cells per wave in two tiles
+// of 16, one launch (k_gate follows).
 // The network runs transposed, Z^T = W H^T, on v_mfma_f32_16x16x32_bf16: W is the A
-// operand (rows = output neurons) and H^T the B operand (columns = the wave's 16
-// cells).  Lane l = 16g + cl holds cell cl; a layer's result tile t gives it neurons
+// operand (rows = output neurons) and H^T the B operand (columns = a tile's 16 cells).
+// Lane l = 16g + cl holds cell cl of each tile; a layer's result tile t gives it neurons
 // 16t + 4g + r (r = 0..3), which, after bias, GELU and rounding to bf16, are the next
 // layer's B operand without leaving registers: k-step s takes tile 2s (j = 0..3) and
 // tile 2s+1 (j = 4..7), and the next layer's weight columns are stored in that order
-// (k_pack_infer, perm = 1).  The weights pass through LDS (infer_layer), the
-// activations never leave registers; the 64 outputs per cell go to out and the Y^0.1
-// of its features to pbuf, both species-major for the gate's coalesced loads.
+// (k_pack_infer, perm = 1).  The weights pass through LDS in groups of output tiles
+// (infer_layer), the activations never leave registers; the 64 outputs per cell go to out
+// and the Y^0.1 of its features to pbuf, both species-major for the gate's coalesced loads.
 struct InferArgs {
   const bf16_t* w[5];  // packed weights: [256][64], 3 x [256][256] (perm), [64][256] (perm)
   const float* b[5];   // biases: 4 x [256], [64]
@@ -477,60 +475,74 @@ struct InferArgs {
   double tstr, log_dt;
 };
 
-// One layer, Z^T = W H^T, for the block's 64 cells.  The block stages each k-step's
-// weight slab (NTILES*16 rows x 32 columns, bf16) through LDS, double-buffered, so the
-// four waves read every weight once from L2 between them; each wave then reads its A
-// fragments (16 bytes a lane, conflict-free) from LDS.  Every thread of the block calls
-// this, once per layer, in the same order.
-template <int NTILES, int KSTEPS>
-__device__ __forceinline__ void infer_layer(const bf16_t* __restrict__ w, const bf16x8* in, f32x4* acc, int cl,
-                                            int g, bf16_t* __restrict__ slab) {
-  // 16-byte chunks of a k-step's slab, and how many each thread stages (the last round
-  // partial when the block has more threads than the slab has chunks)
-  constexpr int K = KSTEPS * 32, ROWS = NTILES * 16, NCH = ROWS * 4, NTH = 64 * NET_WAVES;
-  constexpr int CHUNKS = (NCH + NTH - 1) / NTH;
-  constexpr int BUF = 256 * 32;  // one buffer, bf16 elements
-  const int tid = threadIdx.x;
+// Output tiles T0 .. T0+NT-1 of a layer (rows 16 T0 .. 16 (T0+NT) - 1 of its weights) into
+// the block's LDS: [KSTEPS][16 NT][4] chunks of 16 bytes (8 bf16 of one row and k-step),
+// chunk c of row r at position c ^ ((r >> 2) & 3), so that the 16 lanes one 16-byte LDS
+// read serves at once (rows cl = 0..15 of a tile, one chunk) fall on 16 different bank
+// groups.  The global layout is [ROWS][K] (k_pack_infer).  Every thread of the block
+// calls this; the caller synchronises before and after.
+template <int NT, int KSTEPS>
+__device__ __forceinline__ void stage_tiles(const bf16_t* __restrict__ w, int t0, bf16_t* __restrict__ lds) {
+  constexpr int ROWS = NT * 16, KC = KSTEPS * 4, NCH = ROWS * KC, NTH = 64 * NET_WAVES, R = 4;
+  static_assert(NCH % (NTH * R) == 0, "whole rounds of chunks");
+  const bf16_t* src = w + (int64_t)t0 * 16 * KSTEPS * 32;
+#pragma unroll 1
+  for (int rd = 0; rd < NCH / (NTH * R); ++rd) {
+    bf16x8 v[R];
 #pragma unroll
-  for (int t = 0; t < NTILES; ++t)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc[t][r] = 0.f;
-  bf16x8 ld[CHUNKS];
-#pragma unroll
-  for (int i = 0; i < CHUNKS; ++i) {
-    const int ch = tid + NTH * i, row = ch / 4, col = (ch % 4) * 8;
-    if (NCH % NTH == 0 || ch < NCH) ld[i] = *(const bf16x8*)(w + row * K + col);
-  }
-#pragma unroll
-  for (int i = 0; i < CHUNKS; ++i) {
-    const int ch = tid + NTH * i;
-    if (NCH % NTH == 0 || ch < NCH) *(bf16x8*)(slab + ch * 8) = ld[i];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < KSTEPS; ++s) {
-    const bf16_t* cur = slab + (s % 2) * BUF;
-    if (s + 1 < KSTEPS)
-#pragma unroll
-      for (int i = 0; i < CHUNKS; ++i) {
-        const int ch = tid + NTH * i, row = ch / 4, col = (ch % 4) * 8;
-        if (NCH % NTH == 0 || ch < NCH) ld[i] = *(const bf16x8*)(w + row * K + 32 * (s + 1) + col);
-      }
-#pragma unroll
-    for (int t = 0; t < NTILES; ++t) {
-      const bf16x8 wa = *(const bf16x8*)(cur + (t * 16 + cl) * 32 + 8 * g);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, in[s], acc[t], 0, 0, 0);
+    for (int i = 0; i < R; ++i) {
+      const int ch = (rd * R + i) * NTH + threadIdx.x;   // consecutive threads: consecutive 16 B of a row
+      v[i] = *(const bf16x8*)(src + (int64_t)ch * 8);
     }
-    if (s + 1 < KSTEPS)
 #pragma unroll
-      for (int i = 0; i < CHUNKS; ++i) {
-        const int ch = tid + NTH * i;
-        if (NCH % NTH == 0 || ch < NCH) *(bf16x8*)(slab + ((s + 1) % 2) * BUF + ch * 8) = ld[i];
-      }
+    for (int i = 0; i < R; ++i) {
+      const int ch = (rd * R + i) * NTH + threadIdx.x;
+      const int row = ch / KC, kc = ch % KC, st = kc >> 2, c = kc & 3;
+      *(bf16x8*)(lds + ((st * ROWS + row) * 4 + (c ^ ((row >> 2) & 3))) * 8) = v[i];
+    }
+  }
+}
+
+// Output tiles T0 .. T0+NT-1 of a layer, Z^T = W H^T, for the wave's two tiles of 16
+// cells, from the weights stage_tiles put in LDS: each A fragment (16 bytes a lane) serves
+// both cell tiles' MFMAs.  Per output, the k-steps accumulate in order, so the results do
+// not depend on how the tiles are grouped.
+template <int NT, int KSTEPS>
+__device__ __forceinline__ void tiles_mfma(const bf16_t* __restrict__ lds, int t0, const bf16x8 (*in)[8],
+                                           f32x4 (*acc)[16], int cl, int g) {
+  constexpr int ROWS = NT * 16;
+  const int sw = g ^ ((cl >> 2) & 3);
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[u][t0 + t][r] = 0.f;
+#pragma unroll
+  for (int st = 0; st < KSTEPS; ++st)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const bf16x8 wa = *(const bf16x8*)(lds + ((st * ROWS + t * 16 + cl) * 4 + sw) * 8);
+      acc[0][t0 + t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, in[0][st], acc[0][t0 + t], 0, 0, 0);
+      acc[1][t0 + t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, in[1][st], acc[1][t0 + t], 0, 0, 0);
+    }
+}
+
+// a layer of NTILES output tiles in LDS-sized groups of NT tiles; the block synchronises
+// around each group's staging
+template <int NTILES, int NT, int KSTEPS>
+__device__ __forceinline__ void infer_layer(const bf16_t* __restrict__ w, bf16_t* __restrict__ lds,
+                                            const bf16x8 (*in)[8], f32x4 (*acc)[16], int cl, int g) {
+#pragma unroll
+  for (int t0 = 0; t0 < NTILES; t0 += NT) {
+    stage_tiles<NT, KSTEPS>(w, t0, lds);
+    __syncthreads();
+    tiles_mfma<NT, KSTEPS>(lds, t0, in, acc, cl, g);
     __syncthreads();
   }
 }
 
+// bias, GELU and bf16: a layer's result tiles become the next layer's B operand
 __device__ __forceinline__ void infer_activate(const f32x4* acc, const float* __restrict__ b, int g, bf16x8* act) {
 #pragma unroll
   for (int s = 0; s < 8; ++s)
@@ -541,52 +553,60 @@ __device__ __forceinline__ void infer_activate(const f32x4* acc, const float* __
     }
 }
 
-__global__ void __launch_bounds__(64 * NET_WAVES) k_network(InferArgs a) {
-  __shared__ __attribute__((aligned(16))) bf16_t slab[2 * 256 * 32];  // weight k-steps, double-buffered
+__global__ void __launch_bounds__(64 * NET_WAVES, 2) k_network(InferArgs a) {
+  extern __shared__ __attribute__((aligned(16))) bf16_t wlds[];  // a group of a layer's weight tiles (stage_tiles)
   const int wv = threadIdx.x / 64, l = threadIdx.x % 64, g = l / 16, cl = l % 16;
-  const int64_t q = ((int64_t)blockIdx.x * NET_WAVES + wv) * 16 + cl;
-  const bool live = q < a.ncells;
-  const int64_t c = live ? a.cells[q] : 0;
   const Norm* nm = a.nm;
-  // features: this lane supplies k = 32 s + 8 g + j of its cell
-  bf16x8 x[2];
+  // features: this lane supplies k = 32 s + 8 g + j of cell cl of each of its two tiles
+  bf16x8 act[2][8];
 #pragma unroll
-  for (int s = 0; s < 2; ++s)
+  for (int u = 0; u < 2; ++u) {
+    const int64_t q = ((int64_t)blockIdx.x * NET_WAVES + wv) * 32 + 16 * u + cl;
+    const bool live = q < a.ncells;
+    const int64_t c = live ? a.cells[q] : 0;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int f = 32 * s + 8 * g + j;
-      float v = 0.f;
-      if (live && f < NF) {
-        double raw;
-        if (f >= 2) {
-          const double P = bc_base(a.y[(int64_t)(f - 2) * a.n + c]);
-          a.pbuf[(int64_t)(f - 2) * a.ldo + q] = P;
-          raw = (P - 1.0) / 0.1;
-        } else {
-          raw = f == 0 ? a.tk[c] * a.tstr : a.log_dt;
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int f = 32 * s + 8 * g + j;
+        float v = 0.f;
+        if (live && f < NF) {
+          double raw;
+          if (f >= 2) {
+            const double P = bc_base(a.y[(int64_t)(f - 2) * a.n + c]);
+            a.pbuf[(int64_t)(f - 2) * a.ldo + q] = P;
+            raw = (P - 1.0) / 0.1;
+          } else {
+            raw = f == 0 ? a.tk[c] * a.tstr : a.log_dt;
+          }
+          v = (float)((raw - nm->xmean[f]) / nm->xsd[f]);
         }
-        v = (float)((raw - nm->xmean[f]) / nm->xsd[f]);
+        act[u][s][j] = (__bf16)v;
       }
-      x[s][j] = (__bf16)v;
-    }
-  f32x4 acc[16];
-  bf16x8 act[8];
-  infer_layer<16, 2>(a.w[0], x, acc, cl, g, slab);
-  infer_activate(acc, a.b[0], g, act);
+  }
+  f32x4 acc[2][16];
+  infer_layer<16, 16, 2>(a.w[0], wlds, act, acc, cl, g);
 #pragma unroll 1
   for (int L = 1; L < 4; ++L) {
-    infer_layer<16, 8>(a.w[L], act, acc, cl, g, slab);
-    infer_activate(acc, a.b[L], g, act);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) infer_activate(acc[u], a.b[L - 1], g, act[u]);
+    infer_layer<16, 8, 8>(a.w[L], wlds, act, acc, cl, g);
   }
-  infer_layer<4, 8>(a.w[4], act, acc, cl, g, slab);
-  if (live)
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+  for (int u = 0; u < 2; ++u) infer_activate(acc[u], a.b[3], g, act[u]);
+  infer_layer<4, 4, 8>(a.w[4], wlds, act, acc, cl, g);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int o = 16 * t + 4 * g + r;
-        a.out[o * a.ldo + q] = acc[t][r] + a.b[4][o];
-      }
+  for (int u = 0; u < 2; ++u) {
+    const int64_t q = ((int64_t)blockIdx.x * NET_WAVES + wv) * 32 + 16 * u + cl;
+    if (q < a.ncells)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int o = 16 * t + 4 * g + r;
+          a.out[o * a.ldo + q] = acc[u][t][r] + a.b[4][o];
+        }
+  }
 }
 
 // density of every cell from (Y, T) (main:1056-1061), then the grid's edges: the last
@@ -682,7 +702,9 @@ int us_infer(const uint16_t* const* w, const float* const* b, const void* norm, 
   a.y = y; a.tk = tk; a.out = out; a.pbuf = pbuf; a.ldo = ldo; a.cells = cells; a.n = n; a.ncells = ncells;
   a.tstr = tstr; a.log_dt = log_dt;
   hipStream_t s = (hipStream_t)stream;
-  k_network<<<blocks(ncells, 16 * NET_WAVES), 64 * NET_WAVES, 0, s>>>(a);
+  // the weights' LDS goes beyond the default 64 KB (set on the current device)
+  HIPCHK(hipFuncSetAttribute((const void*)k_network, hipFuncAttributeMaxDynamicSharedMemorySize, NET_LDS));
+  k_network<<<blocks(ncells, 32 * NET_WAVES), 64 * NET_WAVES, NET_LDS, s>>>(a);
   k_gate<true><<<blocks(ncells, GATE_CELLS), GATE_BLOCK, 0, s>>>(out, 1, ldo, pbuf, ldo, (const Norm*)norm, (const Thermo*)thermo, y, tk,
                                               ht, stmf, stnd, n, cells, ncells, log_dt, tol_el, tol_sum, tol_dt,
                                               tol_dy, tol_bc, nullptr, iskip, mask, stats);
