@@ -116,6 +116,11 @@ static_assert(UC_SLAB_SLOTS == SLAB_SLOTS, "slab layout");
 // instructions (1001x401 field, 20 steps: 110.8 against 107.2 ms of sweeps), and at a
 // 64 KB boundary it runs at its best whatever else the code object holds.
 #define UC_HOT __attribute__((aligned(65536)))
+// Waves per SIMD the register allocation must allow, where more waves hide more latency
+// than the spills they cost (1001x401 field, 20 steps): k_sources at 2 (84 B of spills;
+// 12.6 -> 11.1 ms, at 3 it spills 652 B and takes 13.2), k_rates at 3 (324 B; 14.2 ->
+// 13.8 ms), k_finish at 3 (20 B; about 0.5 ms less).
+#define UC_WAVES(n) __attribute__((amdgpu_waves_per_eu(n, n)))
 
 // Reads of the coupled sweep (uc_sweep_cpl): slab slots, the cell's coupling slots
 // (UC_CPL + s, a tile of UC_CPL_SLOTS per wave like the slab), and the neighbours'
@@ -285,7 +290,7 @@ struct RatesArgs {
 
 // rate coefficients, rho*Y_k and soot sources of every cell into its slab
 // (ckrates.f:92-1352)
-__global__ void __launch_bounds__(512) UC_HOT k_rates(const UcParams* __restrict__ Pg, RatesArgs a) {
+__global__ void __launch_bounds__(512) UC_WAVES(3) UC_HOT k_rates(const UcParams* __restrict__ Pg, RatesArgs a) {
   UcP P = (UcP)Pg;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= a.npad) return;  // whole waves only: npad is a multiple of 64
@@ -516,7 +521,7 @@ struct SourcesArgs {
 // and the SOLVEH sources: radiation, RA/PRD, QDOT, QDSP, RHSZ (solveh.f:34-1142).
 // One wave per block: the block's dynamic LDS (UC_PRD_LDS doubles per lane) is the
 // second-level cache of the RA values uc_prd forms.
-__global__ void __launch_bounds__(WAVE) UC_HOT k_sources(const UcParams* __restrict__ Pg, SourcesArgs a) {
+__global__ void __launch_bounds__(WAVE) UC_WAVES(2) UC_HOT k_sources(const UcParams* __restrict__ Pg, SourcesArgs a) {
   extern __shared__ double uc_lds[];
   double* const lds = uc_lds + threadIdx.x;
   UcP P = (UcP)Pg;
@@ -823,7 +828,7 @@ __device__ __forceinline__ void count_fails(const FinishArgs& a, int32_t g, cons
   }
 }
 
-__global__ void __launch_bounds__(512) UC_HOT k_finish(const UcParams* __restrict__ Pg, FinishArgs a) {
+__global__ void __launch_bounds__(512) UC_WAVES(3) UC_HOT k_finish(const UcParams* __restrict__ Pg, FinishArgs a) {
 #pragma clang fp contract(off)   // the reference's operations in its order
   UcP P = (UcP)Pg;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
