@@ -134,7 +134,7 @@ typedef struct {
                            lane in each of that many workgroups, each forming its share of
                            the reactions' slots; with 8 the T(h) kernel also takes 8 lanes
                            per cell (the same bits for every value); 0 (the default): from
-                           the step's cell count, up to 8 for a few tiles */
+                           the step's cell count, 2 for a full field, up to 8 for a few tiles */
 } uc_tuning;
 
 /* What SPGSOLV hands the chemistry beyond the state (spgsolv.f:98-1290); device pointers
