@@ -1368,6 +1368,7 @@ struct Layout {
   bool have_prev = false;        // prev_nsw holds the previous step's sweeps
   bool diag = false;             // coupled step: cells in anti-diagonal order (ctx->cpl_dfirst)
   int32_t ndiag = 0, maxtiles = 0;  // diagonals; most tiles any diagonal's positions touch
+  bool cells_natural = false;    // ctx->cells/gid hold the list in natural order (base_list)
 };
 
 // tuning of a step: the autotuner's choices (uc_tuning)
